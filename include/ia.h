@@ -1,0 +1,168 @@
+/*
+ * ia.h -- C ABI of the MI355X (gfx950) implementation of the two-phase time
+ * step of InterfaceAdvection.jl.
+ *
+ * A Julia host binds these with `ccall` (see the method table in the
+ * integration notes); a Python host binds them with ctypes
+ * (interfaceadvection.jl_amd/__init__.py).  Plain pointers and sizes only.
+ *
+ * Layout of every field buffer handed across this boundary is exactly the
+ * reference's Julia array: column-major, x fastest, one ghost layer per side
+ * (Ng = N + 2), vector components last:
+ *     idx = i + Ngx*(j + Ngy*(k + Ngz*c)),  0 <= i < Ngx, ...
+ * Element type is float (IA_F32) or double (IA_F64) as chosen at creation;
+ * IA_FIELD_CBAR is int8.
+ *
+ * Threading: one context <-> one device <-> one HIP stream; a context is not
+ * re-entrant.  Every call that returns scalars to the host (dt, flags,
+ * solver counts) synchronises its stream before returning.
+ */
+#ifndef IA_H
+#define IA_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ia_ctx ia_ctx;
+
+enum ia_dtype { IA_F32 = 0, IA_F64 = 1 };
+
+/* normalScheme plug-ins (normalEstimation.jl): WH is the reference default
+ * (InterfaceAdvection.jl:70); Column is its first stage. */
+enum ia_normal_scheme { IA_NORMAL_WH = 0, IA_NORMAL_COLUMN = 1 };
+
+/* λ plug-ins (flow.jl:5-15); Koren is the reference default (InterfaceAdvection.jl:70). */
+enum ia_limiter { IA_LIMITER_KOREN = 0, IA_LIMITER_MINMOD = 1, IA_LIMITER_UPWIND = 2 };
+
+/* Pressure solve.
+ * REFERENCE: WaterLily solver!(ml; tol, itmx) exactly as the reference calls it
+ *            (flow.jl:346): stop when Σr² <= tol (absolute) or after itmx V-cycles.
+ * REFINED:   FP32 MultiLevelPoisson V-cycles inside an FP64 iterative-refinement
+ *            loop; stop when ||z - A x||2 / ||z||2 <= rtol (FP64 residual, FP64
+ *            pressure accumulation).  No reference counterpart (mixed precision).
+ * LEARNED:   REFINED, with the initial guess from the learned correction network
+ *            (no reference counterpart); the FP64 residual is the acceptance test. */
+enum ia_poisson_mode { IA_POISSON_REFERENCE = 0, IA_POISSON_REFINED = 1, IA_POISSON_LEARNED = 2 };
+
+/* Status codes (advection.jl:145-189 reportFillError and the glue's errors). */
+enum ia_status {
+    IA_OK = 0,
+    IA_ERR_NAN = 1,        /* error("NaN!")                         advection.jl:148 */
+    IA_ERR_EXPLODE = 2,    /* error("divergence ... is exploding!")  advection.jl:160,:180 */
+    IA_WARN_FILL = 3,      /* fill error printed, not thrown          advection.jl:161-166 */
+    IA_ERR_HIP = 4,        /* HIP runtime error */
+    IA_ERR_BADARG = 5      /* bad argument / unsupported plug-in */
+};
+
+/* Field ids (the cVOF and Flow arrays of cVOF.jl:18-43 and WaterLily's Flow). */
+enum ia_field {
+    IA_FIELD_F = 0,      /* cVOF.f   (s) */
+    IA_FIELD_F0 = 1,     /* cVOF.f⁰  (s) */
+    IA_FIELD_ALPHA = 2,  /* cVOF.α   (s) -- materialised by ia_reconstruct */
+    IA_FIELD_NHAT = 3,   /* cVOF.n̂   (v) -- materialised by ia_reconstruct */
+    IA_FIELD_CBAR = 5,   /* cVOF.c̄   (s, int8) */
+    IA_FIELD_RHOU = 6,   /* cVOF.ρu  (v) */
+    IA_FIELD_RHOUF = 7,  /* cVOF.ρuf (v) */
+    IA_FIELD_U = 9,      /* Flow.u   (v) */
+    IA_FIELD_U0 = 10,    /* Flow.u⁰  (v) */
+    IA_FIELD_FORCE = 11, /* Flow.f   (v) -- written by ia_visc */
+    IA_FIELD_P = 12,     /* Flow.p   (s) */
+    IA_FIELD_SIGMA = 13, /* Flow.σ   (s) -- Poisson source z */
+    IA_FIELD_MU0 = 14    /* Flow.μ₀  (v) -- Poisson coefficients L */
+};
+
+typedef struct ia_params {
+    int D;                 /* 2 or 3 */
+    int N[3];              /* interior cells per axis; MultiLevelPoisson needs N+2 = a*2^n, n>2 */
+    int dtype;             /* enum ia_dtype */
+    int perdir;            /* bit d set -> axis d (0-based) periodic */
+    double u_bc[3];        /* constant u_BC tuple */
+    double g[3];           /* constant gravity, g(i,x,t) = g[i] */
+    int has_g;             /* 0 -> g === nothing */
+    double nu;             /* Flow.ν; cVOF.μ = ν (0 -> inviscid, cVOF.jl:78) */
+    double lambda_rho;     /* λρ light/dark density ratio */
+    double lambda_mu;      /* λμ light/dark viscosity ratio */
+    double eta;            /* surface tension; must be 0 (η == nothing) */
+    int normal_scheme;     /* enum ia_normal_scheme */
+    int limiter;           /* enum ia_limiter */
+    int poisson_mode;      /* enum ia_poisson_mode */
+    double poisson_tol;    /* REFERENCE: tol on Σr² (reference 1e-4) */
+    int poisson_itmx;      /* REFERENCE: itmx (reference 200) */
+    double poisson_rtol;   /* REFINED/LEARNED: relative FP64 residual (e.g. 1e-6) */
+    int poisson_max_outer; /* REFINED/LEARNED: outer iteration cap */
+    int device;            /* HIP device ordinal */
+} ia_params;
+
+typedef struct ia_solve_stats {
+    int iters;             /* V-cycles (REFERENCE) or outer refinement steps (REFINED) */
+    double r2;             /* final Σr² of the FP32/T solve (REFERENCE) */
+    double rel;            /* final FP64 relative residual (REFINED/LEARNED), else 0 */
+    double ms;             /* device time of the solve, milliseconds */
+} ia_solve_stats;
+
+typedef struct ia_step_stats {
+    double dt_next;        /* min(MPCFL, 1.2 dt)  (flow.jl:108) */
+    int status;            /* worst status over the step's sweeps */
+    int fill_warnings;     /* number of sweeps that reported a fill error */
+    ia_solve_stats solve[2]; /* predictor, corrector projections */
+} ia_step_stats;
+
+/* TwoPhaseSimulation / cVOF construction (InterfaceAdvection.jl:67-85,
+ * cVOF.jl:45-89, WaterLily Flow/MultiLevelPoisson): allocates every field
+ * with the reference's initial values (u = u_BC, f = 1, dρ = 1, μ₀ = 1 + BC). */
+int ia_create(const ia_params *p, ia_ctx **out);
+void ia_destroy(ia_ctx *c);
+
+/* Copy a whole field in/out (nbytes must equal the field's size). */
+int ia_set_field(ia_ctx *c, int field, const void *host, size_t nbytes);
+int ia_get_field(ia_ctx *c, int field, void *host, size_t nbytes);
+/* Zero-copy access to the current device buffer of a field. */
+int ia_field_device_ptr(ia_ctx *c, int field, void **dptr, size_t *nbytes);
+/* Sizes of MultiLevelPoisson level l (ghost-inclusive); returns the level count. */
+int ia_mg_levels(ia_ctx *c, int l, int *Ng);
+
+/* applyVOF!(f,α,n̂,sdf) + cleanWisp! + BCf! + f⁰ = copy(f)  (VOFutil.jl:8-37,
+ * cVOF.jl:64-65).  `sdf` holds, per interior cell in x-fastest order, the
+ * 1+2D samples [sdf(x), sdf(x+Δx e1), sdf(x-Δx e1), ...] at x = I .- 1.5,
+ * Δx = 0.01 (the SDF is a host closure, evaluated by the caller). */
+int ia_apply_vof(ia_ctx *c, const void *sdf, size_t nbytes);
+/* WaterLily Flow: apply!(uλ,u); BC!(u); u⁰ = copy(u).  `u` is the full
+ * ghost-inclusive field of uλ(i, loc(i,I)) samples. */
+int ia_set_initial_velocity(ia_ctx *c, const void *u, size_t nbytes);
+
+/* MPCFL(flow, intf)  (flow.jl:262-281). */
+int ia_cfl(ia_ctx *c, double *dt);
+
+/* sim_step!(::TwoPhaseSimulation) -> MPFMomStep!(flow,pois,intf,body; δt=dt)
+ * (InterfaceAdvection.jl:100-103, flow.jl:60-109).  step_index = length(Δt)
+ * selects the sweep order (flow.jl:163).  stats->dt_next is the Δt to push. */
+int ia_mom_step(ia_ctx *c, double dt, int step_index, ia_step_stats *stats);
+
+/* advect!(flow, intf, f, u⁰, u, dt)  (advection.jl:17-23, :34-78): pure
+ * cVOF advection of f with the average of u⁰ and u. */
+int ia_advect(ia_ctx *c, double dt, int step_index, ia_step_stats *stats);
+
+/* Stage entry points (each one reference routine, for callers that drive the
+ * step themselves and for stage-level parity tests). */
+/* reconstructInterface!(f,α,n̂,normalScheme)  normalEstimation.jl:10-28 */
+int ia_reconstruct(ia_ctx *c);
+/* c̄ = f≥½; ρuf = 0; advectVOF1d!(f,…,u⁰,u,dt,d)  advection.jl:80-89 (d is 1-based) */
+int ia_vof_sweep(ia_ctx *c, int d, double dt, ia_step_stats *stats);
+/* viscSurfTenρu!(Flow.f, u, …, f or f⁰)  flow.jl:113-139 */
+int ia_visc(ia_ctx *c, int use_f0);
+/* fill!(μ₀,1); updateL!(μ₀, f or f⁰)  flow.jl:254-259 */
+int ia_update_L(ia_ctx *c, int use_f0);
+/* update!(::MultiLevelPoisson)  (WaterLily; flow.jl:81) */
+int ia_mg_update(ia_ctx *c);
+/* myproject!(flow,pois,w) with dt_w = w*Δt, then BC!(u)  flow.jl:328-347, :82 */
+int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats);
+
+int ia_synchronize(ia_ctx *c);
+const char *ia_last_error(ia_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

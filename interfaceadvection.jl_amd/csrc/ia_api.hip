@@ -1,0 +1,851 @@
+// ia_api.hip -- context, step orchestration and the C ABI (include/ia.h).
+//
+// The host side only sequences kernels on the context's stream; every field
+// lives in HBM for the life of the context.  Scalars cross to the host once
+// per pressure V-cycle (the solver's stopping test) and once per step (CFL
+// and fill flags).
+#include "ia.h"
+#include "ia_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+using namespace ia;
+
+namespace {
+
+struct Level {
+    Grid g;
+    Tiling t;
+    void *L = nullptr, *Dg = nullptr, *iD = nullptr, *x = nullptr, *e = nullptr, *r = nullptr, *z = nullptr;
+    bool own = false;
+};
+
+}  // namespace
+
+struct ia_ctx {
+    ia_params P;
+    int es = 4;
+    Grid g;
+    Tiling t;
+    int dev = 0;
+    hipStream_t s = nullptr;
+    // fields
+    void *fb[4] = {};   // pool of scalar buffers for f, f⁰ and sweep outputs
+    int fi = 0, f0i = 1;
+    void *u = nullptr, *u0 = nullptr, *ru = nullptr, *ru2 = nullptr, *ruf = nullptr, *F = nullptr;
+    void *p = nullptr, *sig = nullptr, *L = nullptr, *al = nullptr, *nh = nullptr;
+    int8_t *cb = nullptr;
+    Level lev[12];
+    int nlev = 0;
+    // reductions and device-side control
+    FillPart *fpart = nullptr;
+    double *part = nullptr;
+    long part_n = 0;
+    StepState *st = nullptr;
+    SolveCtl *ctl = nullptr;
+    double *dsc = nullptr;   // device scalars
+    double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    char msg[512] = {0};
+};
+
+namespace {
+
+int fail(ia_ctx *c, int code, const char *fmt, ...) {
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->msg, sizeof c->msg, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+#define HIPCHK(c, x)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (x);                                                                \
+        if (_e != hipSuccess) return fail(c, IA_ERR_HIP, "%s: %s", #x, hipGetErrorString(_e)); \
+    } while (0)
+#define LAUNCHCHK(c)                                                                              \
+    do {                                                                                          \
+        hipError_t _e = hipGetLastError();                                                        \
+        if (_e != hipSuccess) return fail(c, IA_ERR_HIP, "launch: %s", hipGetErrorString(_e));    \
+    } while (0)
+
+void grid_init(Grid &g, int D, const int *Ng, int perdir) {
+    g.D = D;
+    for (int a = 0; a < 3; ++a) {
+        g.Ng[a] = a < D ? Ng[a] : 1;
+        g.per[a] = a < D ? ((perdir >> a) & 1) : 0;
+    }
+    g.st[0] = 1;
+    g.st[1] = g.Ng[0];
+    g.st[2] = (long)g.Ng[0] * g.Ng[1];
+    g.ns = (long)g.Ng[0] * g.Ng[1] * g.Ng[2];
+}
+Tiling tiling(const Grid &g) {
+    Tiling t;
+    int n0 = g.Ng[0] - 2, n1 = g.D > 1 ? g.Ng[1] - 2 : 1, n2 = g.D > 2 ? g.Ng[2] - 2 : 1;
+    t.nt[0] = (n0 + TX - 1) / TX;
+    t.nt[1] = (n1 + TY - 1) / TY;
+    t.nt[2] = n2;
+    t.tz = 1;
+    t.nblk = (long)t.nt[0] * t.nt[1] * t.nt[2];
+    return t;
+}
+long interior(const Grid &g) {
+    long n = 1;
+    for (int a = 0; a < g.D; ++a) n *= g.Ng[a] - 2;
+    return n;
+}
+inline dim3 blocks(const Tiling &t) { return dim3((unsigned)t.nblk); }
+inline dim3 blocks1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+size_t field_elems(ia_ctx *c, int id) {
+    long ns = c->g.ns, D = c->g.D;
+    switch (id) {
+    case IA_FIELD_F: case IA_FIELD_F0: case IA_FIELD_ALPHA: case IA_FIELD_CBAR: case IA_FIELD_P: case IA_FIELD_SIGMA:
+        return ns;
+    case IA_FIELD_NHAT: case IA_FIELD_RHOU: case IA_FIELD_RHOUF: case IA_FIELD_U: case IA_FIELD_U0:
+    case IA_FIELD_FORCE: case IA_FIELD_MU0:
+        return ns * D;
+    default:
+        return 0;
+    }
+}
+void *field_buf(ia_ctx *c, int id) {
+    switch (id) {
+    case IA_FIELD_F: return c->fb[c->fi];
+    case IA_FIELD_F0: return c->fb[c->f0i];
+    case IA_FIELD_ALPHA: return c->al;
+    case IA_FIELD_NHAT: return c->nh;
+    case IA_FIELD_CBAR: return c->cb;
+    case IA_FIELD_RHOU: return c->ru;
+    case IA_FIELD_RHOUF: return c->ruf;
+    case IA_FIELD_U: return c->u;
+    case IA_FIELD_U0: return c->u0;
+    case IA_FIELD_FORCE: return c->F;
+    case IA_FIELD_P: return c->p;
+    case IA_FIELD_SIGMA: return c->sig;
+    case IA_FIELD_MU0: return c->L;
+    default: return nullptr;
+    }
+}
+int free_fbuf(ia_ctx *c, int a, int b, int d2) {
+    for (int q = 0; q < 4; ++q)
+        if (q != a && q != b && q != d2) return q;
+    return -1;
+}
+
+template <class T> T *P(void *v) { return (T *)v; }
+template <class T> const T *P(const void *v) { return (const T *)v; }
+
+// WaterLily BC!(a, A) on a staggered vector field whose interior is current
+template <class T>
+__global__ void __launch_bounds__(NT) k_bcvel(Grid g, Tiling t, T *a, T d0, T d1, T d2) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T dv[3] = {d0, d1, d2};
+    for (int i = 0; i < g.D; ++i) write_images<T>(g, a + g.ns * i, p, a[g.ns * i + I], 1, i, dv[i]);
+}
+
+// ---------------------------------------------------------------------------
+// typed implementation
+// ---------------------------------------------------------------------------
+template <class T> struct Impl {
+    static double epsT() { return sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16; }
+
+    static int alloc_zero(ia_ctx *c, void **p, size_t bytes) {
+        HIPCHK(c, hipMalloc(p, bytes));
+        HIPCHK(c, hipMemsetAsync(*p, 0, bytes, c->s));
+        return IA_OK;
+    }
+    static int fill(ia_ctx *c, void *p, long n, T v) {
+        hipLaunchKernelGGL(k_fill<T>, blocks1(n), dim3(256), 0, c->s, P<T>(p), n, v);
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static int bc_vel(ia_ctx *c, void *a, const T *dv) {
+        hipLaunchKernelGGL(k_bcvel<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(a), dv[0], dv[1], dv[2]);
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+
+    static int create(ia_ctx *c) {
+        Grid &g = c->g;
+        size_t ns = g.ns, nv = ns * g.D, es = sizeof(T);
+        for (int q = 0; q < 4; ++q)
+            if (int r = alloc_zero(c, &c->fb[q], ns * es)) return r;
+        void **vec[] = {&c->u, &c->u0, &c->ru, &c->ru2, &c->ruf, &c->F, &c->L, &c->nh};
+        for (void **v : vec)
+            if (int r = alloc_zero(c, v, nv * es)) return r;
+        void **sca[] = {&c->p, &c->sig, &c->al};
+        for (void **v : sca)
+            if (int r = alloc_zero(c, v, ns * es)) return r;
+        HIPCHK(c, hipMalloc(&c->cb, ns));
+        HIPCHK(c, hipMemsetAsync(c->cb, 0, ns, c->s));
+        // Flow: u = u_BC, BC!, u⁰ = u ; μ₀ = 1, BC!(μ₀, 0)
+        T ub[3], z3[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) ub[a] = (T)c->P.u_bc[a];
+        for (int d = 0; d < g.D; ++d) fill(c, P<T>(c->u) + ns * d, ns, ub[d]);
+        bc_vel(c, c->u, ub);
+        HIPCHK(c, hipMemcpyAsync(c->u0, c->u, nv * es, hipMemcpyDeviceToDevice, c->s));
+        fill(c, c->L, nv, T(1));
+        bc_vel(c, c->L, z3);
+        // cVOF: f = 1, f⁰ = 1
+        fill(c, c->fb[0], ns, T(1));
+        fill(c, c->fb[1], ns, T(1));
+        c->fi = 0;
+        c->f0i = 1;
+        // MultiLevelPoisson(p, μ₀, σ)
+        Level &l0 = c->lev[0];
+        l0.g = g;
+        l0.t = c->t;
+        l0.L = c->L;
+        l0.x = c->p;
+        l0.z = c->sig;
+        void **l0b[] = {&l0.Dg, &l0.iD, &l0.e, &l0.r};
+        for (void **v : l0b)
+            if (int r = alloc_zero(c, v, ns * es)) return r;
+        c->nlev = 1;
+        long maxblk = c->t.nblk;
+        for (;;) {
+            Grid &gp = c->lev[c->nlev - 1].g;
+            bool ok = true;
+            for (int a = 0; a < g.D; ++a) ok = ok && gp.Ng[a] % 2 == 0 && gp.Ng[a] > 4;
+            if (!(ok && c->nlev <= 10)) break;
+            int Na[3];
+            for (int a = 0; a < 3; ++a) Na[a] = a < g.D ? 1 + gp.Ng[a] / 2 : 1;
+            Level &lv = c->lev[c->nlev];
+            grid_init(lv.g, g.D, Na, c->P.perdir);
+            lv.t = tiling(lv.g);
+            size_t n = lv.g.ns;
+            if (int r = alloc_zero(c, &lv.L, n * g.D * es)) return r;
+            void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z};
+            for (void **v : lb)
+                if (int r = alloc_zero(c, v, n * es)) return r;
+            lv.own = true;
+            maxblk = std::max(maxblk, lv.t.nblk);
+            c->nlev++;
+        }
+        if (c->nlev <= 2) return fail(c, IA_ERR_BADARG, "MultiLevelPoisson requires size=a2^n, where n>2");
+        c->part_n = 2 * maxblk + 64;
+        HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
+        HIPCHK(c, hipMalloc(&c->fpart, maxblk * sizeof(FillPart)));
+        HIPCHK(c, hipMalloc(&c->st, sizeof(StepState)));
+        HIPCHK(c, hipMalloc(&c->ctl, sizeof(SolveCtl)));
+        HIPCHK(c, hipMalloc(&c->dsc, 64 * sizeof(double)));
+        HIPCHK(c, hipMemsetAsync(c->dsc, 0, 64 * sizeof(double), c->s));
+        if (int r = mg_update(c)) return r;
+        return IA_OK;
+    }
+
+    // ----------------------------------------------------------------- MG
+    static int mg_update(ia_ctx *c) {
+        for (int l = 0; l < c->nlev; ++l) {
+            Level &lv = c->lev[l];
+            if (l > 0) {
+                Level &lf = c->lev[l - 1];
+                hipLaunchKernelGGL(k_restrictL<T>, blocks(lv.t), dim3(NT), 0, c->s, lv.g, lv.t, lf.g, P<T>(lf.L),
+                                   P<T>(lv.L));
+            }
+            hipLaunchKernelGGL(k_set_diag<T>, blocks(lv.t), dim3(NT), 0, c->s, lv.g, lv.t, P<T>(lv.L), P<T>(lv.Dg),
+                               P<T>(lv.iD));
+        }
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static void finsum(ia_ctx *c, long nblk, double *out) {
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, c->s, c->part, nblk, out);
+    }
+    static void jacobi(ia_ctx *c, Level &l) {
+        hipLaunchKernelGGL(k_jacobi_eps<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.r), P<T>(l.iD), P<T>(l.e));
+        increment(c, l);
+    }
+    static void increment(ia_ctx *c, Level &l) {
+        hipLaunchKernelGGL(k_increment<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.L), P<T>(l.Dg), P<T>(l.e),
+                           P<T>(l.r), P<T>(l.x));
+    }
+    static void pcg(ia_ctx *c, Level &l, int it = 6) {
+        long nb = l.t.nblk;
+        hipLaunchKernelGGL(k_pcg_init<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.r), P<T>(l.iD), P<T>(l.z),
+                           P<T>(l.e), c->part);
+        hipLaunchKernelGGL(k_pcg_fin_rho0<T>, dim3(1), dim3(NT), 0, c->s, c->part, nb, c->ctl);
+        for (int q = 1; q <= it; ++q) {
+            hipLaunchKernelGGL(k_pcg_Ae<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.L), P<T>(l.Dg), P<T>(l.e),
+                               P<T>(l.z), c->part, c->ctl);
+            hipLaunchKernelGGL(k_pcg_fin_alpha<T>, dim3(1), dim3(NT), 0, c->s, c->part, nb, c->ctl, it);
+            hipLaunchKernelGGL(k_pcg_update<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.iD), P<T>(l.e),
+                               P<T>(l.z), P<T>(l.r), P<T>(l.x), c->part, c->ctl);
+            hipLaunchKernelGGL(k_pcg_fin_beta<T>, dim3(1), dim3(NT), 0, c->s, c->part, nb, c->ctl);
+            if (q < it)
+                hipLaunchKernelGGL(k_pcg_eps<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.z), P<T>(l.e), c->ctl);
+        }
+    }
+    static void vcycle(ia_ctx *c, int l) {
+        Level &fl = c->lev[l], &co = c->lev[l + 1];
+        jacobi(c, fl);
+        hipLaunchKernelGGL(k_restrict<T>, blocks(co.t), dim3(NT), 0, c->s, co.g, co.t, fl.g, P<T>(fl.r), P<T>(co.r),
+                           P<T>(co.x));
+        if (l + 2 < c->nlev) vcycle(c, l + 1);
+        pcg(c, co);
+        hipLaunchKernelGGL(k_prolong<T>, blocks(fl.t), dim3(NT), 0, c->s, fl.g, fl.t, co.g, P<T>(co.x), P<T>(fl.e));
+        increment(c, fl);
+    }
+    static int l2(ia_ctx *c, Level &l, double *out_host) {
+        hipLaunchKernelGGL(k_dot<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.r), P<T>(l.r), c->part);
+        finsum(c, l.t.nblk, c->dsc + 0);
+        HIPCHK(c, hipMemcpyAsync(out_host, c->dsc + 0, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        *out_host = (double)(T)*out_host;
+        return IA_OK;
+    }
+    // WaterLily solver!(ml; tol, itmx)
+    static int solver(ia_ctx *c, double tol, int itmx, int *iters, double *r2o) {
+        Level &l0 = c->lev[0];
+        hipLaunchKernelGGL(k_residual<T>, blocks(l0.t), dim3(NT), 0, c->s, l0.g, l0.t, P<T>(l0.L), P<T>(l0.Dg),
+                           P<T>(l0.iD), P<T>(l0.x), P<T>(l0.z), P<T>(l0.r), c->part);
+        hipLaunchKernelGGL(k_fin_mean<T>, dim3(1), dim3(NT), 0, c->s, c->part, l0.t.nblk, interior(l0.g), c->ctl);
+        hipLaunchKernelGGL(k_sub_mean<T>, blocks(l0.t), dim3(NT), 0, c->s, l0.g, l0.t, P<T>(l0.r), c->ctl);
+        LAUNCHCHK(c);
+        double r2;
+        if (int e = l2(c, l0, &r2)) return e;
+        int n = 0;
+        while (r2 > tol && n < itmx) {
+            vcycle(c, 0);
+            pcg(c, l0);
+            LAUNCHCHK(c);
+            if (int e = l2(c, l0, &r2)) return e;
+            n++;
+        }
+        hipLaunchKernelGGL(k_perbc<T>, blocks(l0.t), dim3(NT), 0, c->s, l0.g, l0.t, P<T>(l0.x));
+        LAUNCHCHK(c);
+        *iters = n;
+        *r2o = r2;
+        return IA_OK;
+    }
+    // myproject! (flow.jl:328-347) + BC!(u), REFERENCE mode
+    static int project_ref(ia_ctx *c, void *u, double dtw_d, ia_solve_stats *ss) {
+        Grid &g = c->g;
+        T dtw = (T)dtw_d;
+        hipLaunchKernelGGL(k_div<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->sig));
+        hipLaunchKernelGGL(k_scale<T>, blocks1(g.ns), dim3(256), 0, c->s, P<T>(c->p), g.ns, dtw);
+        int it = 0;
+        double r2 = 0;
+        if (int e = solver(c, c->P.poisson_tol, c->P.poisson_itmx, &it, &r2)) return e;
+        hipLaunchKernelGGL(k_project<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->L), P<T>(c->p),
+                           (T)c->P.u_bc[0], (T)c->P.u_bc[1], (T)c->P.u_bc[2]);
+        hipLaunchKernelGGL(k_scale<T>, blocks1(g.ns), dim3(256), 0, c->s, P<T>(c->p), g.ns, T(1) / dtw);
+        LAUNCHCHK(c);
+        if (ss) {
+            ss->iters = it;
+            ss->r2 = r2;
+            ss->rel = 0;
+        }
+        return IA_OK;
+    }
+    // FP64 iterative refinement around the V-cycle (REFINED / LEARNED)
+    static int resid64(ia_ctx *c, double *rn2_host) {
+        Level &l0 = c->lev[0];
+        Grid &g = c->g;
+        long nb = c->t.nblk;
+        hipLaunchKernelGGL(k_ir_resid<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(l0.L), P<T>(l0.iD), c->z64,
+                           c->x64, c->r64, c->part);
+        finsum(c, nb, c->dsc + 2);
+        hipLaunchKernelGGL(k_ir_center, blocks(c->t), dim3(NT), 0, c->s, g, c->t, c->r64, c->dsc + 2, interior(g),
+                           c->part);
+        finsum(c, nb, c->dsc + 3);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(rn2_host, c->dsc + 3, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return IA_OK;
+    }
+    static int project_refined(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
+        Grid &g = c->g;
+        Level &l0 = c->lev[0];
+        long nb = c->t.nblk, ncell = interior(g);
+        if (!c->x64) {
+            HIPCHK(c, hipMalloc(&c->x64, g.ns * sizeof(double)));
+            HIPCHK(c, hipMalloc(&c->r64, g.ns * sizeof(double)));
+            HIPCHK(c, hipMalloc(&c->z64, g.ns * sizeof(double)));
+            HIPCHK(c, hipMemsetAsync(c->x64, 0, g.ns * sizeof(double), c->s));
+            HIPCHK(c, hipMemsetAsync(c->r64, 0, g.ns * sizeof(double), c->s));
+            HIPCHK(c, hipMemsetAsync(c->z64, 0, g.ns * sizeof(double), c->s));
+        }
+        hipLaunchKernelGGL(k_ir_init<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->p), dtw, c->z64,
+                           c->x64, c->part);
+        finsum(c, nb, c->dsc + 4);
+        hipLaunchKernelGGL(k_ir_center, blocks(c->t), dim3(NT), 0, c->s, g, c->t, c->z64, c->dsc + 4, ncell, c->part);
+        finsum(c, nb, c->dsc + 5);
+        double zn2 = 0, rn2 = 0;
+        HIPCHK(c, hipMemcpyAsync(&zn2, c->dsc + 5, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        if (int e = resid64(c, &rn2)) return e;
+        double rel = zn2 > 0 ? sqrt(rn2 / zn2) : 0.0;
+        int it = 0;
+        while (rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
+            hipLaunchKernelGGL(k_ir_load<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, c->r64, c->dsc + 3, ncell,
+                               P<T>(l0.r), P<T>(l0.x));
+            vcycle(c, 0);
+            pcg(c, l0);
+            hipLaunchKernelGGL(k_ir_accum<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(l0.x), c->dsc + 3, ncell,
+                               c->x64);
+            LAUNCHCHK(c);
+            if (int e = resid64(c, &rn2)) return e;
+            rel = sqrt(rn2 / zn2);
+            it++;
+        }
+        hipLaunchKernelGGL(k_ir_project<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->L), c->x64,
+                           P<T>(c->p), dtw, (T)c->P.u_bc[0], (T)c->P.u_bc[1], (T)c->P.u_bc[2]);
+        LAUNCHCHK(c);
+        if (ss) {
+            ss->iters = it;
+            ss->r2 = rn2;
+            ss->rel = rel;
+        }
+        return IA_OK;
+    }
+    static int project(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
+        HIPCHK(c, hipEventRecord(c->e0, c->s));
+        int e = c->P.poisson_mode == IA_POISSON_REFERENCE ? project_ref(c, u, dtw, ss) : project_refined(c, u, dtw, ss);
+        if (e) return e;
+        HIPCHK(c, hipEventRecord(c->e1, c->s));
+        if (ss) {
+            HIPCHK(c, hipEventSynchronize(c->e1));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
+            ss->ms = ms;
+        }
+        return IA_OK;
+    }
+
+    // ----------------------------------------------------------------- CFL
+    static int cfl(ia_ctx *c, double *out) {
+        Grid &g = c->g;
+        hipLaunchKernelGGL(k_cfl<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(c->u), c->part);
+        hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(NT), 0, c->s, c->part, c->t.nblk, c->dsc + 8);
+        LAUNCHCHK(c);
+        double mx[2];
+        HIPCHK(c, hipMemcpyAsync(mx, c->dsc + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        T nu = (T)c->P.nu, lm = (T)c->P.lambda_mu, lr = (T)c->P.lambda_rho;
+        T dtAdv = T(1) / ((T)mx[0] + 5 * nu);
+        T dtV = T(1) / (2 * (T)mx[1]);
+        T dtG = 1;
+        if (c->P.has_g) {
+            T g2 = 0;
+            for (int d = 0; d < g.D; ++d) g2 += (T)c->P.g[d] * (T)c->P.g[d];
+            dtG = T(1) / (2 * (T)sqrt(g2));
+        }
+        T dtVisc = c->P.nu != 0 ? T(3) / ((14 * nu) * std::max(T(1), lm / lr)) : T(1);
+        T m = std::min(std::min(std::min(std::min(dtV, dtAdv), dtG), dtVisc), T(1));
+        *out = (double)(T(0.8) * m);
+        return IA_OK;
+    }
+
+    // ----------------------------------------------------------------- sweeps
+    static void prepare(ia_ctx *c, const void *f, const void *usrc, bool do_ru) {
+        hipLaunchKernelGGL(k_prepare<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(f), P<T>(usrc), P<T>(c->ru),
+                           c->cb, (T)c->P.lambda_rho, do_ru ? 1 : 0);
+    }
+    static void vof_sweep(ia_ctx *c, const void *fin, void *fout, const void *u1, const void *u2, int d, T dt,
+                          int mode, double rtol, const void *fn, void *fmid) {
+        Grid &g = c->g;
+        SweepArgs<T> a;
+        a.g = g;
+        a.t = c->t;
+        a.fin = P<T>((void *)fin);
+        a.fout = P<T>(fout);
+        a.u1 = P<T>((void *)u1) + g.ns * d;
+        a.u2 = P<T>((void *)u2) + g.ns * d;
+        a.cb = c->cb;
+        a.ruf = P<T>(c->ruf) + g.ns * d;
+        a.fn = P<T>((void *)fn);
+        a.fmid = P<T>(fmid);
+        a.part = c->fpart;
+        a.d = d;
+        a.scheme = c->P.normal_scheme;
+        a.mode = mode;
+        a.dt = dt;
+        a.lr = (T)c->P.lambda_rho;
+        a.idt = T(1) / dt;
+        a.rtol = (T)rtol;
+        a.ctol = 10 * (T)epsT();
+        a.ubc_d = (T)c->P.u_bc[d];
+        hipLaunchKernelGGL(k_vof_sweep<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        hipLaunchKernelGGL(k_fill_finalize<T>, dim3(1), dim3(NT), 0, c->s, g, c->fpart, c->t.nblk, P<T>((void *)u1),
+                           P<T>((void *)u2), rtol, c->st);
+    }
+    static void momentum(ia_ctx *c, const void *fpre, const void *u1, const void *u2, const void *uold, int d, T dt) {
+        Grid &g = c->g;
+        MomArgs<T> a;
+        a.g = g;
+        a.t = c->t;
+        a.ru_in = P<T>(c->ru);
+        a.ru_out = P<T>(c->ru2);
+        a.fpre = P<T>((void *)fpre);
+        a.ruf = P<T>(c->ruf) + g.ns * d;
+        a.cb = c->cb;
+        a.u1 = P<T>((void *)u1) + g.ns * d;
+        a.u2 = P<T>((void *)u2) + g.ns * d;
+        a.uold = P<T>((void *)uold);
+        a.d = d;
+        a.lim = c->P.limiter;
+        a.dt = dt;
+        a.lr = (T)c->P.lambda_rho;
+        for (int q = 0; q < 3; ++q) a.ubc[q] = (T)c->P.u_bc[q];
+        hipLaunchKernelGGL(k_momentum<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        std::swap(c->ru, c->ru2);
+    }
+    static void dir_order(int D, int step_index, int *dirO) {
+        for (int i = 1; i <= D; ++i) dirO[i - 1] = ((step_index + i) % D);  // 0-based (flow.jl:163)
+    }
+    // advectfq!/advectVOFρuu! (flow.jl:157-210): advect pool buffer `fsrc`;
+    // returns the pool index holding the result; the last sweep also writes
+    // f_mid = (f* + f^n)/2 into pool index `mid` when mid >= 0.
+    static int advect_fq(ia_ctx *c, int fsrc, const void *u1, const void *u2, const void *uold, T dt, int step_index,
+                         int keep, int mid) {
+        int dirO[3];
+        dir_order(c->g.D, step_index, dirO);
+        int cur = fsrc;
+        int a = free_fbuf(c, fsrc, keep, mid), b = -1;
+        for (int q = 0; q < 4; ++q)
+            if (q != fsrc && q != keep && q != mid && q != a) { b = q; break; }
+        int outs[2] = {a, b};
+        double rtol = 100 * epsT();
+        for (int op = 0; op < c->g.D; ++op) {
+            int d = dirO[op];
+            int out = outs[op & 1];
+            bool last = op == c->g.D - 1;
+            vof_sweep(c, c->fb[cur], c->fb[out], u1, u2, d, dt, 0, rtol, last && mid >= 0 ? c->fb[keep] : nullptr,
+                      last && mid >= 0 ? c->fb[mid] : nullptr);
+            momentum(c, c->fb[cur], u1, u2, uold, d, dt);
+            cur = out;
+        }
+        return cur;
+    }
+    static int upd(ia_ctx *c, const void *ust, const void *u0, const void *fcur, const void *fupd, void *uout, T dt,
+                   T w) {
+        Grid &g = c->g;
+        UpdArgs<T> a;
+        a.v.g = g;
+        a.v.t = c->t;
+        a.v.u = P<T>((void *)ust);
+        a.v.fv = P<T>((void *)fupd);
+        a.v.lm = (T)c->P.lambda_mu;
+        a.v.mu = (T)c->P.nu;
+        a.v.lr = (T)c->P.lambda_rho;
+        for (int q = 0; q < 3; ++q) a.v.ubc[q] = (T)c->P.u_bc[q];
+        a.v.has_mu = c->P.nu != 0;
+        a.u0 = P<T>((void *)u0);
+        a.fcur = P<T>((void *)fcur);
+        a.fupd = P<T>((void *)fupd);
+        a.ru = P<T>(c->ru);
+        a.uout = P<T>(uout);
+        a.L = P<T>(c->L);
+        a.dt = dt;
+        a.w = w;
+        for (int q = 0; q < 3; ++q) a.g[q] = c->P.has_g ? (T)c->P.g[q] : T(0);
+        hipLaunchKernelGGL(k_visc_update<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static int read_state(ia_ctx *c, ia_step_stats *ss) {
+        StepState h;
+        HIPCHK(c, hipMemcpyAsync(&h, c->st, sizeof h, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (ss) {
+            ss->status = h.status;
+            ss->fill_warnings = h.nwarn;
+        }
+        if (h.status == IA_ERR_NAN) fail(c, IA_ERR_NAN, "NaN!");
+        else if (h.status == IA_ERR_EXPLODE) fail(c, IA_ERR_EXPLODE, "divergence is exploding! (VOF %.6g / %.6g)", h.maxf, h.minf);
+        else if (h.status == IA_WARN_FILL)
+            fail(c, IA_WARN_FILL, "VOF not in [0,1]: max %.9g @ %ld, min %.9g @ %ld", h.maxf, h.imax, h.minf, h.imin);
+        return h.status;
+    }
+
+    // MPFMomStep! (flow.jl:60-109)
+    static int mom_step(ia_ctx *c, double dtd, int step_index, ia_step_stats *ss) {
+        Grid &g = c->g;
+        T dt = (T)dtd;
+        HIPCHK(c, hipMemsetAsync(c->st, 0, sizeof(StepState), c->s));
+        int fn = c->fi;
+        void *U0 = c->u, *U1 = c->u0;  // u⁰ ← u: the predictor reads u^n from U0 and writes u^{n+1/2} to U1
+        // predictor (w = ½)
+        T w = T(0.5);
+        prepare(c, c->fb[fn], U0, true);
+        int mid = free_fbuf(c, fn, -1, -1);
+        int fstar = advect_fq(c, fn, U0, U0, U0, dt, step_index, fn, mid);
+        (void)fstar;
+        if (int e = upd(c, U0, U0, c->fb[fn], c->fb[mid], U1, dt, w)) return e;
+        if (int e = mg_update(c)) return e;
+        if (int e = project(c, U1, (double)(w * dt), ss ? &ss->solve[0] : nullptr)) return e;
+        // corrector (w = 1): f⁰ ← f; ρu = u⁰ρ(f); advect f with u^{n+½}
+        prepare(c, c->fb[fn], U0, true);
+        int fnew = advect_fq(c, fn, U1, U1, U0, dt, step_index, fn, -1);
+        if (int e = upd(c, U1, U0, c->fb[fnew], c->fb[fnew], U0, dt, T(1))) return e;
+        if (int e = mg_update(c)) return e;
+        if (int e = project(c, U0, (double)dt, ss ? &ss->solve[1] : nullptr)) return e;
+        c->u = U0;
+        c->u0 = U1;
+        c->fi = fnew;
+        c->f0i = fn;
+        double cf;
+        if (int e = cfl(c, &cf)) return e;
+        int st = read_state(c, ss);
+        double nx = std::min(cf, 1.2 * (double)dt);
+        if (ss) ss->dt_next = (double)(T)nx;
+        return st;
+    }
+    // advect! (advection.jl:17-78)
+    static int advect(ia_ctx *c, double dtd, int step_index, ia_step_stats *ss) {
+        Grid &g = c->g;
+        T dt = (T)dtd;
+        HIPCHK(c, hipMemsetAsync(c->st, 0, sizeof(StepState), c->s));
+        HIPCHK(c, hipMemsetAsync(c->ruf, 0, g.ns * g.D * sizeof(T), c->s));
+        prepare(c, c->fb[c->fi], c->u0, false);
+        int dirO[3];
+        dir_order(g.D, step_index, dirO);
+        int cur = c->fi;
+        for (int op = 0; op < g.D; ++op) {
+            int d = dirO[op];
+            int out = free_fbuf(c, cur, c->f0i, -1);
+            vof_sweep(c, c->fb[cur], c->fb[out], c->u0, c->u, d, dt, 1, 10 * epsT(), nullptr, nullptr);
+            cur = out;
+        }
+        LAUNCHCHK(c);
+        c->fi = cur;
+        int st = read_state(c, ss);
+        if (ss) ss->dt_next = dtd;
+        return st;
+    }
+    static int vof_sweep_stage(ia_ctx *c, int d, double dtd, ia_step_stats *ss) {
+        Grid &g = c->g;
+        HIPCHK(c, hipMemsetAsync(c->st, 0, sizeof(StepState), c->s));
+        HIPCHK(c, hipMemsetAsync(c->ruf, 0, g.ns * g.D * sizeof(T), c->s));
+        prepare(c, c->fb[c->fi], c->u0, false);
+        int out = free_fbuf(c, c->fi, c->f0i, -1);
+        vof_sweep(c, c->fb[c->fi], c->fb[out], c->u0, c->u, d, (T)dtd, 1, 100 * epsT(), nullptr, nullptr);
+        LAUNCHCHK(c);
+        c->fi = out;
+        return read_state(c, ss);
+    }
+    static int visc_stage(ia_ctx *c, int use_f0) {
+        ViscArgs<T> a;
+        a.g = c->g;
+        a.t = c->t;
+        a.u = P<T>(c->u);
+        a.fv = P<T>(c->fb[use_f0 ? c->f0i : c->fi]);
+        a.lm = (T)c->P.lambda_mu;
+        a.mu = (T)c->P.nu;
+        a.lr = (T)c->P.lambda_rho;
+        for (int q = 0; q < 3; ++q) a.ubc[q] = (T)c->P.u_bc[q];
+        a.has_mu = c->P.nu != 0;
+        hipLaunchKernelGGL(k_visc_force<T>, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->F));
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static int update_L_stage(ia_ctx *c, int use_f0) {
+        hipLaunchKernelGGL(k_update_L<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t,
+                           P<T>(c->fb[use_f0 ? c->f0i : c->fi]), P<T>(c->L), (T)c->P.lambda_rho);
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static int reconstruct_stage(ia_ctx *c) {
+        hipLaunchKernelGGL(k_reconstruct<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(c->fb[c->fi]),
+                           P<T>(c->al), P<T>(c->nh), c->P.normal_scheme);
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static int apply_vof(ia_ctx *c, const void *sdf, size_t nbytes) {
+        Grid &g = c->g;
+        size_t need = (size_t)interior(g) * (1 + 2 * g.D) * sizeof(T);
+        if (nbytes != need) return fail(c, IA_ERR_BADARG, "ia_apply_vof: %zu bytes, expected %zu", nbytes, need);
+        void *d = nullptr;
+        HIPCHK(c, hipMalloc(&d, need));
+        HIPCHK(c, hipMemcpyAsync(d, sdf, need, hipMemcpyHostToDevice, c->s));
+        hipLaunchKernelGGL(k_apply_vof<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(d), P<T>(c->fb[c->fi]),
+                           P<T>(c->al), P<T>(c->nh), 10 * (T)epsT());
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(c->fb[c->f0i], c->fb[c->fi], g.ns * sizeof(T), hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        HIPCHK(c, hipFree(d));
+        return IA_OK;
+    }
+    static int set_initial_velocity(ia_ctx *c, const void *u, size_t nbytes) {
+        Grid &g = c->g;
+        size_t need = g.ns * g.D * sizeof(T);
+        if (nbytes != need) return fail(c, IA_ERR_BADARG, "ia_set_initial_velocity: %zu bytes, expected %zu", nbytes, need);
+        HIPCHK(c, hipMemcpyAsync(c->u, u, need, hipMemcpyHostToDevice, c->s));
+        T ub[3];
+        for (int a = 0; a < 3; ++a) ub[a] = (T)c->P.u_bc[a];
+        if (int e = bc_vel(c, c->u, ub)) return e;
+        HIPCHK(c, hipMemcpyAsync(c->u0, c->u, need, hipMemcpyDeviceToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return IA_OK;
+    }
+};
+
+template <class F> int dispatch(ia_ctx *c, F &&f) {
+    if (!c) return IA_ERR_BADARG;
+    HIPCHK(c, hipSetDevice(c->dev));
+    return c->P.dtype == IA_F64 ? f(Impl<double>()) : f(Impl<float>());
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int ia_create(const ia_params *p, ia_ctx **out) {
+    if (!p || !out) return IA_ERR_BADARG;
+    *out = nullptr;
+    if (!(p->D == 2 || p->D == 3) || !(p->dtype == IA_F32 || p->dtype == IA_F64)) return IA_ERR_BADARG;
+    for (int a = 0; a < p->D; ++a)
+        if (p->N[a] < 2) return IA_ERR_BADARG;
+    if (p->eta != 0) return IA_ERR_BADARG;  // surface tension is out of scope (η == nothing)
+    if (p->normal_scheme < 0 || p->normal_scheme > 1 || p->limiter < 0 || p->limiter > 2) return IA_ERR_BADARG;
+    if (p->poisson_mode != IA_POISSON_REFERENCE && p->dtype != IA_F32) return IA_ERR_BADARG;
+    ia_ctx *c = new ia_ctx();
+    c->P = *p;
+    if (c->P.poisson_tol <= 0) c->P.poisson_tol = 1e-4;
+    if (c->P.poisson_itmx <= 0) c->P.poisson_itmx = 200;
+    if (c->P.poisson_rtol <= 0) c->P.poisson_rtol = 1e-6;
+    if (c->P.poisson_max_outer <= 0) c->P.poisson_max_outer = 100;
+    c->dev = p->device;
+    c->es = p->dtype == IA_F64 ? 8 : 4;
+    int Ng[3];
+    for (int a = 0; a < 3; ++a) Ng[a] = a < p->D ? p->N[a] + 2 : 1;
+    grid_init(c->g, p->D, Ng, p->perdir);
+    c->t = tiling(c->g);
+    if (hipSetDevice(c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+        delete c;
+        return IA_ERR_HIP;
+    }
+    int r = dispatch(c, [&](auto impl) { return decltype(impl)::create(c); });
+    if (r == IA_OK) r = hipStreamSynchronize(c->s) == hipSuccess ? IA_OK : IA_ERR_HIP;
+    if (r != IA_OK) {
+        fprintf(stderr, "ia_create: %s\n", c->msg);
+        ia_destroy(c);
+        return r;
+    }
+    *out = c;
+    return IA_OK;
+}
+
+void ia_destroy(ia_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->dev);
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    void *bufs[] = {c->fb[0], c->fb[1], c->fb[2], c->fb[3], c->u, c->u0, c->ru, c->ru2, c->ruf, c->F, c->p, c->sig,
+                    c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->ctl, c->dsc, c->x64, c->r64, c->z64};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (int l = 0; l < c->nlev; ++l) {
+        Level &lv = c->lev[l];
+        void *lb[] = {lv.Dg, lv.iD, lv.e, lv.r};
+        for (void *b : lb)
+            if (b) (void)hipFree(b);
+        if (lv.own) {
+            void *ob[] = {lv.L, lv.x, lv.z};
+            for (void *b : ob)
+                if (b) (void)hipFree(b);
+        }
+    }
+    if (c->e0) (void)hipEventDestroy(c->e0);
+    if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+}
+
+int ia_set_field(ia_ctx *c, int field, const void *host, size_t nbytes) {
+    if (!c) return IA_ERR_BADARG;
+    size_t n = field_elems(c, field) * (field == IA_FIELD_CBAR ? 1 : c->es);
+    void *b = field_buf(c, field);
+    if (!b || n != nbytes) return fail(c, IA_ERR_BADARG, "ia_set_field(%d): %zu bytes, expected %zu", field, nbytes, n);
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMemcpyAsync(b, host, n, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return IA_OK;
+}
+int ia_get_field(ia_ctx *c, int field, void *host, size_t nbytes) {
+    if (!c) return IA_ERR_BADARG;
+    size_t n = field_elems(c, field) * (field == IA_FIELD_CBAR ? 1 : c->es);
+    void *b = field_buf(c, field);
+    if (!b || n != nbytes) return fail(c, IA_ERR_BADARG, "ia_get_field(%d): %zu bytes, expected %zu", field, nbytes, n);
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMemcpyAsync(host, b, n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return IA_OK;
+}
+int ia_field_device_ptr(ia_ctx *c, int field, void **dptr, size_t *nbytes) {
+    if (!c || !dptr) return IA_ERR_BADARG;
+    void *b = field_buf(c, field);
+    if (!b) return fail(c, IA_ERR_BADARG, "unknown field %d", field);
+    *dptr = b;
+    if (nbytes) *nbytes = field_elems(c, field) * (field == IA_FIELD_CBAR ? 1 : c->es);
+    return IA_OK;
+}
+int ia_mg_levels(ia_ctx *c, int l, int *Ng) {
+    if (!c) return IA_ERR_BADARG;
+    if (Ng && l >= 0 && l < c->nlev)
+        for (int a = 0; a < 3; ++a) Ng[a] = c->lev[l].g.Ng[a];
+    return c->nlev;
+}
+int ia_apply_vof(ia_ctx *c, const void *sdf, size_t nbytes) {
+    return dispatch(c, [&](auto impl) { return decltype(impl)::apply_vof(c, sdf, nbytes); });
+}
+int ia_set_initial_velocity(ia_ctx *c, const void *u, size_t nbytes) {
+    return dispatch(c, [&](auto impl) { return decltype(impl)::set_initial_velocity(c, u, nbytes); });
+}
+int ia_cfl(ia_ctx *c, double *dt) {
+    if (!dt) return IA_ERR_BADARG;
+    return dispatch(c, [&](auto impl) { return decltype(impl)::cfl(c, dt); });
+}
+int ia_mom_step(ia_ctx *c, double dt, int step_index, ia_step_stats *stats) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::mom_step(c, dt, step_index, stats); });
+}
+int ia_advect(ia_ctx *c, double dt, int step_index, ia_step_stats *stats) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::advect(c, dt, step_index, stats); });
+}
+int ia_reconstruct(ia_ctx *c) {
+    return dispatch(c, [&](auto impl) { return decltype(impl)::reconstruct_stage(c); });
+}
+int ia_vof_sweep(ia_ctx *c, int d, double dt, ia_step_stats *stats) {
+    if (!c || d < 1 || d > c->g.D) return IA_ERR_BADARG;
+    if (stats) memset(stats, 0, sizeof *stats);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::vof_sweep_stage(c, d - 1, dt, stats); });
+}
+int ia_visc(ia_ctx *c, int use_f0) {
+    return dispatch(c, [&](auto impl) { return decltype(impl)::visc_stage(c, use_f0); });
+}
+int ia_update_L(ia_ctx *c, int use_f0) {
+    return dispatch(c, [&](auto impl) { return decltype(impl)::update_L_stage(c, use_f0); });
+}
+int ia_mg_update(ia_ctx *c) {
+    return dispatch(c, [&](auto impl) { return decltype(impl)::mg_update(c); });
+}
+int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::project(c, c->u, dt_w, stats); });
+}
+int ia_synchronize(ia_ctx *c) {
+    if (!c) return IA_ERR_BADARG;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return IA_OK;
+}
+const char *ia_last_error(ia_ctx *c) { return c ? c->msg : "null context"; }
+
+}  // extern "C"

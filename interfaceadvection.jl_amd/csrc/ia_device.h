@@ -1,0 +1,346 @@
+// ia_device.h -- device-side building blocks for the gfx950 two-phase step.
+//
+// Coordinates inside kernels are 0-based and ghost-inclusive: p[a] in
+// [0, Ng[a]), interior 1..Ng-2 (the reference's 2..Ng-1).  Arithmetic in
+// every function follows the reference expression by expression (and the
+// same order of operations), so that FP32 results equal the FP32 CPU oracle
+// up to the last-ulp differences of the transcendental functions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ia {
+
+struct Grid {
+    int D;
+    int Ng[3];    // ghost-inclusive extents (1 on an absent axis)
+    int per[3];   // periodic flags
+    long st[3];   // strides
+    long ns;      // cells
+};
+
+// ---------------------------------------------------------------------------
+// launch geometry: one thread per interior cell; 1-D grid of tiles whose
+// block id is swizzled so that each XCD (8 on MI355X, blocks dispatched
+// round-robin) works on one contiguous eighth of the domain, keeping stencil
+// neighbours in the same L2.
+// ---------------------------------------------------------------------------
+constexpr int TX = 64, TY = 4;
+struct Tiling {
+    int nt[3];   // tiles per axis over the interior
+    int tz;      // cells per tile along z (1)
+    long nblk;
+};
+__device__ inline long swizzle_block(long b, long nblk) {
+    const long nx = 8;
+    long per = nblk / nx;
+    if (per == 0) return b;
+    long full = per * nx;
+    if (b >= full) return b;
+    return (b % nx) * per + b / nx;
+}
+// interior cell of this thread (returns false when outside the interior)
+__device__ inline bool thread_cell(const Grid &g, const Tiling &t, int p[3]) {
+    long b = swizzle_block((long)blockIdx.x, t.nblk);
+    long bx = b % t.nt[0];
+    long r = b / t.nt[0];
+    long by = r % t.nt[1];
+    long bz = r / t.nt[1];
+    int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    p[0] = 1 + (int)bx * TX + tx;
+    p[1] = g.D > 1 ? 1 + (int)by * TY + ty : 0;
+    p[2] = g.D > 2 ? 1 + (int)bz : 0;
+    bool ok = p[0] <= g.Ng[0] - 2;
+    if (g.D > 1) ok = ok && p[1] <= g.Ng[1] - 2;
+    if (g.D > 2) ok = ok && p[2] <= g.Ng[2] - 2;
+    return ok;
+}
+__device__ inline long lin(const Grid &g, const int p[3]) {
+    return (long)p[0] + g.st[1] * p[1] + g.st[2] * p[2];
+}
+
+// ---------------------------------------------------------------------------
+// boundary maps.  Reading through these equals reading the reference's array
+// after its BC routine ran.
+// ---------------------------------------------------------------------------
+// BCf! / perBC! axis map for cell-centred scalars (VOFutil.jl:64-74)
+__device__ inline int map_scalar(const Grid &g, int a, int q) {
+    int n = g.Ng[a];
+    if (a >= g.D) return q;
+    if (q == 0) return g.per[a] ? n - 2 : 1;
+    if (q == n - 1) return g.per[a] ? 1 : n - 2;
+    return q;
+}
+__device__ inline long lin_scalar(const Grid &g, int p0, int p1, int p2) {
+    return (long)map_scalar(g, 0, p0) + g.st[1] * map_scalar(g, 1, p1) + g.st[2] * map_scalar(g, 2, p2);
+}
+// periodic-only map (perBC! on Poisson iterates)
+__device__ inline int map_per(const Grid &g, int a, int q) {
+    int n = g.Ng[a];
+    if (a >= g.D || !g.per[a]) return q;
+    if (q == 0) return n - 2;
+    if (q == n - 1) return 1;
+    return q;
+}
+// WaterLily BC!(a, A, false, perdir) on component c: true -> Dirichlet A[c]
+__device__ inline bool vel_dirichlet(const Grid &g, int c, const int p[3]) {
+    int n = g.Ng[c];
+    return !g.per[c] && (p[c] == 0 || p[c] == 1 || p[c] == n - 1);
+}
+__device__ inline long lin_vel(const Grid &g, int c, const int p[3]) {
+    int q[3];
+    for (int a = 0; a < 3; ++a) q[a] = map_scalar(g, a, p[a]);
+    return (long)q[0] + g.st[1] * q[1] + g.st[2] * q[2];
+}
+// BCv! on a face-averaged field, component c (VOFutil.jl:91-104): returns
+// false when the slot is one f2face!/BCv! never writes (p_c = Ng-1 on a
+// non-periodic axis), whose stale value the caller supplies.
+__device__ inline bool face_map(const Grid &g, int c, const int p[3], int q[3]) {
+    for (int a = 0; a < 3; ++a) {
+        int n = g.Ng[a];
+        int v = p[a];
+        if (a < g.D) {
+            if (g.per[a]) {
+                if (v == 0) v = n - 2;
+                else if (v == n - 1) v = 1;
+            } else if (a == c) {
+                if (v == n - 1) return false;
+                if (v == 0) v = 2;
+            } else {
+                if (v == 0) v = 1;
+                else if (v == n - 1) v = n - 2;
+            }
+        }
+        q[a] = v;
+    }
+    return true;
+}
+
+// write an interior value and every ghost image of it.  kind 0: scalar
+// (BCf!); kind 1: velocity-like component c (BC! with Dirichlet value dval).
+template <class T>
+__device__ inline void write_images(const Grid &g, T *a, const int p[3], T val, int kind, int c, T dval) {
+    int alt[3][2], na[3];
+    bool dir[3][2];
+    for (int ax = 0; ax < 3; ++ax) {
+        na[ax] = 1;
+        alt[ax][0] = p[ax];
+        dir[ax][0] = false;
+        if (ax >= g.D) continue;
+        int n = g.Ng[ax];
+        bool vel_axis = kind == 1 && ax == c && !g.per[ax];
+        if (vel_axis && p[ax] == 1) dir[ax][0] = true;
+        int img = -1;
+        if (g.per[ax]) {
+            if (p[ax] == 1) img = n - 1;
+            else if (p[ax] == n - 2) img = 0;
+        } else {
+            if (p[ax] == 1) img = 0;
+            else if (p[ax] == n - 2) img = n - 1;
+        }
+        if (img >= 0) {
+            alt[ax][1] = img;
+            dir[ax][1] = vel_axis;
+            na[ax] = 2;
+        }
+        // a 1-cell interior (n == 3) has both images
+        if (n == 3 && !g.per[ax]) {
+            // p == 1 == n-2: images 0 and 2; handled by the second pass below
+        }
+    }
+    for (int i2 = 0; i2 < na[2]; ++i2)
+        for (int i1 = 0; i1 < na[1]; ++i1)
+            for (int i0 = 0; i0 < na[0]; ++i0) {
+                bool d = dir[0][i0] || dir[1][i1] || dir[2][i2];
+                long idx = (long)alt[0][i0] + g.st[1] * alt[1][i1] + g.st[2] * alt[2][i2];
+                a[idx] = d ? dval : val;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// scalar helpers with Julia semantics
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline T tmin(T a, T b) { return b < a ? b : a; }
+template <class T> __device__ inline T tmax(T a, T b) { return b > a ? b : a; }
+template <class T> __device__ inline T tsign(T x) { return x > 0 ? T(1) : (x < 0 ? T(-1) : x); }
+template <class T> __device__ inline T linp(T f, T lam) { return lam + (T(1) - lam) * f; }  // VOFutil.jl:166
+template <class T> __device__ inline bool fullorempty(T f) { return f == T(0) || f == T(1); }
+template <class T> __device__ inline T median3(T a, T b, T c) { return tmax(tmin(a, b), tmin(tmax(a, b), c)); }
+template <class T> __device__ inline T eps_of();
+template <> __device__ inline float eps_of<float>() { return 1.1920928955078125e-07f; }
+template <> __device__ inline double eps_of<double>() { return 2.220446049250313e-16; }
+
+// ---------------------------------------------------------------------------
+// PLIC (PLIC.jl, Scardovelli & Zaleski 2000)
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline void sort2(T &a, T &b) {
+    if (!(a < b)) { T t = a; a = b; b = t; }
+}
+template <class T> __device__ inline void sort3(T &a, T &b, T &c) {
+    T t;
+    if (a > c) { t = a; a = c; c = t; }
+    if (a > b) { t = a; a = b; b = t; }
+    if (b > c) { t = b; b = c; c = t; }
+}
+template <class T> __device__ inline T proot(T c0, T c1, T c2, T c3) {  // PLIC.jl:165-176
+    T a0 = c0 / c3, a1 = c1 / c3, a2 = c2 / c3;
+    T p0 = a1 / 3 - (a2 * a2) / 9;
+    T q0 = (a1 * a2 - 3 * a0) / 6 - (a2 * a2 * a2) / 27;
+    T a = q0 / sqrt(-(p0 * p0 * p0));
+    T t = acos((a * a) <= T(1) ? a : T(0)) / 3;
+    return sqrt(-p0) * (sqrt(T(3)) * sin(t) - cos(t)) - a2 / 3;
+}
+template <class T> __device__ inline T a2f2(T m1, T m2, T a) {  // PLIC.jl:92
+    return a < m1 ? (a * a) / ((2 * m1) * m2) : (a - m1 / 2) / m2;
+}
+template <class T> __device__ inline T a2f3(T m1, T m2, T m3, T a) {  // PLIC.jl:93-107
+    T m12 = m1 + m2;
+    if (a < m1) return (a * a * a) / (((6 * m1) * m2) * m3);
+    if (a < m2) return (a * (a - m1)) / ((2 * m2) * m3) + ((m2 == T(0) ? T(1) : m1 / m2) * m1) / (6 * m3);
+    if (a < tmin(m3, m12))
+        return ((a * a) * (3 * m12 - a) + (m1 * m1) * (m1 - 3 * a) + (m2 * m2) * (m2 - 3 * a)) / (((6 * m1) * m2) * m3);
+    if (m3 < m12)
+        return ((a * a) * (3 - 2 * a) + (m1 * m1) * (m1 - 3 * a) + (m2 * m2) * (m2 - 3 * a) + (m3 * m3) * (m3 - 3 * a)) /
+               (((6 * m1) * m2) * m3);
+    return (2 * a - m12) / (2 * m3);
+}
+template <class T> __device__ inline T f2a2(T m1, T m2, T v) {  // PLIC.jl:117
+    return v < m1 / (2 * m2) ? sqrt(((2 * m1) * m2) * v) : m2 * v + m1 / 2;
+}
+template <class T> __device__ inline T f2a3(T m1, T m2, T m3, T v) {  // PLIC.jl:118-149
+    T m12 = m1 + m2;
+    T p = ((6 * m1) * m2) * m3;
+    T v1 = ((m2 == T(0) ? T(1) : m1 / m2) * m1) / (6 * m3);
+    T v2 = v1 + (m2 - m1) / (2 * m3);
+    T v3 = m3 < m12 ? ((m3 * m3) * (3 * m12 - m3) + (m1 * m1) * (m1 - 3 * m3) + (m2 * m2) * (m2 - 3 * m3)) / p
+                    : m12 / (2 * m3);
+    if (v < v1) return cbrt(p * v);
+    if (v < v2) return (m1 + sqrt(m1 * m1 + ((8 * m2) * m3) * (v - v1))) / 2;
+    if (v < v3) {
+        T c0 = (m1 * m1 * m1 + m2 * m2 * m2) - p * v;
+        T c1 = -3 * (m1 * m1 + m2 * m2);
+        T c2 = 3 * m12;
+        return proot(c0, c1, c2, T(-1));
+    }
+    if (m3 < m12) {
+        T c0 = ((m1 * m1 * m1 + m2 * m2 * m2) + m3 * m3 * m3) - p * v;
+        T c1 = -3 * ((m1 * m1 + m2 * m2) + m3 * m3);
+        return proot(c0, c1, T(3), T(-2));
+    }
+    return m3 * v + m12 / 2;
+}
+template <class T> __device__ inline T getIntercept(int D, const T *n, T g) {  // PLIC.jl:20-39
+    T a;
+    if (D == 2) {
+        T t = fabs(n[0]) + fabs(n[1]);
+        if (g != T(0.5)) {
+            T m1 = fabs(n[0]) / t, m2 = fabs(n[1]) / t;
+            sort2(m1, m2);
+            a = f2a2(m1, m2, g < T(0.5) ? g : 1 - g);
+        } else
+            a = T(0.5);
+        return ((g < T(0.5) ? a : 1 - a) * t + tmin(n[0], T(0))) + tmin(n[1], T(0));
+    }
+    T t = (fabs(n[0]) + fabs(n[1])) + fabs(n[2]);
+    if (g != T(0.5)) {
+        T m1 = fabs(n[0]) / t, m2 = fabs(n[1]) / t, m3 = fabs(n[2]) / t;
+        sort3(m1, m2, m3);
+        a = f2a3(m1, m2, m3, g < T(0.5) ? g : 1 - g);
+    } else
+        a = T(0.5);
+    return (((g < T(0.5) ? a : 1 - a) * t + tmin(n[0], T(0))) + tmin(n[1], T(0))) + tmin(n[2], T(0));
+}
+template <class T> __device__ inline T getVolumeFraction(int D, const T *n, T b) {  // PLIC.jl:59-82
+    if (D == 2) {
+        T t = fabs(n[0]) + fabs(n[1]);
+        T a = ((b - tmin(n[0], T(0))) - tmin(n[1], T(0))) / t;
+        if (a <= T(0) || a == T(0.5) || a >= T(1)) return tmin(tmax(a, T(0)), T(1));
+        T m1 = fabs(n[0]) / t, m2 = fabs(n[1]) / t;
+        sort2(m1, m2);
+        T v = a2f2(m1, m2, a < T(0.5) ? a : 1 - a);
+        return a < T(0.5) ? v : 1 - v;
+    }
+    T t = (fabs(n[0]) + fabs(n[1])) + fabs(n[2]);
+    T a = (((b - tmin(n[0], T(0))) - tmin(n[1], T(0))) - tmin(n[2], T(0))) / t;
+    if (a <= T(0) || a == T(0.5) || a >= T(1)) return tmin(tmax(a, T(0)), T(1));
+    T m1 = fabs(n[0]) / t, m2 = fabs(n[1]) / t, m3 = fabs(n[2]) / t;
+    sort3(m1, m2, m3);
+    T v = a2f3(m1, m2, m3, a < T(0.5) ? a : 1 - a);
+    return a < T(0.5) ? v : 1 - v;
+}
+
+// ---------------------------------------------------------------------------
+// interface normals (normalEstimation.jl)
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline T h3(const Grid &g, const T *f, long I, int s) {  // VOFutil.jl:158
+    return (f[I] + f[I - g.st[s]]) + f[I + g.st[s]];
+}
+template <class T> __device__ inline int argabsmax(int D, const T *n) {  // util.jl:19-31 (0-based)
+    T mx = 0;
+    int im = 0;
+    for (int i = 0; i < D; ++i) {
+        T cur = n[i] * n[i];
+        if (cur > mx) { mx = cur; im = i; }
+    }
+    return im;
+}
+// normalEstimation.jl:75-97 (Column) ; I is a cell whose 3^D block is valid
+template <class T> __device__ inline void normal_column(const Grid &g, const T *f, long I, T *n) {
+    for (int d = 0; d < g.D; ++d) n[d] = f[I - g.st[d]] - f[I + g.st[d]];
+    int dom = argabsmax(g.D, n);
+    for (int d = 0; d < g.D; ++d) {
+        if (d == dom) {
+            T s = tsign(n[d]);
+            n[d] = s == T(0) ? T(1) : s;
+            continue;
+        }
+        T hl = h3(g, f, I - g.st[d], dom), hr = h3(g, f, I + g.st[d], dom);
+        n[d] = (hl - hr) / 2;
+    }
+}
+// normalEstimation.jl:105-154 (WH, the default).  Float64 literals of the
+// Julia source stay double here.
+template <class T> __device__ inline void normal_wh(const Grid &g, const T *f, long I, T *n) {
+    normal_column(g, f, I, n);
+    int im = argabsmax(g.D, n);
+    int sdom = signbit(n[im]) ? -1 : 1;
+    T an = fabs(n[im]);
+    an = an == T(0) ? T(1) : an;
+    for (int d = 0; d < g.D; ++d) n[d] /= an;
+    for (int d = 0; d < g.D; ++d) {
+        if (d == im) continue;
+        T slope = fabs(n[d]);
+        int scur = signbit(n[d]) ? -1 : 1;
+        long off = scur * g.st[d];
+        T hl = h3(g, f, I - off, im), hc = h3(g, f, I, im), hr = h3(g, f, I + off, im);
+        T sh = (hl + hc) + hr;
+        double s45 = 4.5 * (double)slope;
+        if (s45 <= (double)sh && (double)sh <= 9 - s45) continue;
+        long offd = sdom * g.st[im];
+        double r1 = (double)(T(1) / (2 * slope));
+        double mn = s45 < r1 ? s45 : r1;
+        if ((double)sh < s45) {
+            T wb = h3(g, f, I - offd, d);
+            if ((double)wb > mn) n[d] = (T)copysign(((double)hl - 0.5) / ((double)wb - 0.5), (double)scur);
+        }
+        if ((double)sh > 9 - s45) {
+            T wt = h3(g, f, I + offd, d);
+            if ((double)wt < 3 - mn) n[d] = (T)copysign((2.5 - (double)hr) / (2.5 - (double)wt), (double)scur);
+        }
+    }
+}
+template <class T> __device__ inline void normal_scheme(int scheme, const Grid &g, const T *f, long I, T *n) {
+    if (scheme == 1) normal_column(g, f, I, n);
+    else normal_wh(g, f, I, n);
+}
+
+// ---------------------------------------------------------------------------
+// limiters (flow.jl:5-7)
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline T limiter(int lim, T u, T c, T d) {
+    if (lim == 1) return median3((3 * c - u) / 2, c, (c + d) / 2);  // minmod
+    if (lim == 2) return c;                                         // upwind
+    return median3(((7 * c + d) - 2 * u) / 6, c, median3(2 * c - u, c, d));  // Koren
+}
+
+}  // namespace ia

@@ -1,0 +1,945 @@
+// ia_kernels.h -- kernels of the two-phase step (templated on the field type).
+#pragma once
+#include "ia_device.h"
+
+namespace ia {
+
+constexpr int NT = TX * TY;  // threads per block (256 = 4 wave64)
+
+// ---------------------------------------------------------------------------
+// block reductions (deterministic: fixed shuffle tree, fixed LDS order)
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double block_sum(double v) {
+    __shared__ double sh[NT / 64];
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NT / 64; ++q) s += sh[q];
+    return s;  // valid in thread 0
+}
+__device__ inline double block_max(double v) {
+    __shared__ double sh[NT / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        double w2 = __shfl_xor(v, o, 64);
+        v = w2 > v ? w2 : v;
+    }
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = sh[0];
+    if (threadIdx.x == 0)
+        for (int q = 1; q < NT / 64; ++q) s = sh[q] > s ? sh[q] : s;
+    return s;
+}
+
+// argmax/argmin partials of a VOF sweep (findmax/findmin, advection.jl:146-147)
+struct FillPart {
+    double vmax, vmin;
+    long imax, imin;
+    int nan, pad;
+};
+__device__ inline void fill_combine(FillPart &a, const FillPart &b) {
+    if (b.vmax > a.vmax || (b.vmax == a.vmax && b.imax < a.imax)) { a.vmax = b.vmax; a.imax = b.imax; }
+    if (b.vmin < a.vmin || (b.vmin == a.vmin && b.imin < a.imin)) { a.vmin = b.vmin; a.imin = b.imin; }
+    a.nan |= b.nan;
+}
+__device__ inline FillPart block_fill(FillPart v) {
+    __shared__ FillPart sh[NT];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) fill_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// device-resident state of a step (flags) and of a solve (pcg scalars)
+struct StepState {
+    int status, nwarn;
+    double maxf, minf;
+    long imax, imin;
+};
+struct SolveCtl {
+    double rho, alpha, beta, s, r2, dot;
+    int done, stop, it, pad;
+};
+
+// ---------------------------------------------------------------------------
+// c̄ = f≥½ (advection.jl:40, flow.jl:172) and ρu = u·ρ(f) (VOFutil.jl:208-211)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(NT) k_prepare(Grid g, Tiling t, const T *__restrict__ f, const T *__restrict__ usrc,
+                                                 T *__restrict__ ru, int8_t *__restrict__ cb, T lr, int do_ru) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    int8_t c = f[I] < T(0.5) ? 0 : 1;
+    write_images<int8_t>(g, cb, p, c, 0, 0, 0);
+    if (do_ru)
+        for (int d = 0; d < g.D; ++d) {
+            long o = g.ns * d;
+            ru[o + I] = usrc[o + I] * linp((f[I] + f[I - g.st[d]]) / 2, lr);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// one cVOF sweep along d: reconstruction of every donor cell on the fly,
+// swept-volume face fluxes, conservative update with the c̄ dilation term,
+// fill report, cleanWisp!, BCf! (advection.jl:80-137, normalEstimation.jl:10-28)
+// ---------------------------------------------------------------------------
+template <class T> struct SweepArgs {
+    Grid g;
+    Tiling t;
+    const T *fin;
+    T *fout;
+    const T *u1, *u2;    // face velocities, component d
+    const int8_t *cb;
+    T *ruf;              // mass flux, component d
+    const T *fn;         // f^n for the mid-point f (predictor) or null
+    T *fmid;
+    FillPart *part;
+    int d, scheme, mode; // mode 0: MPFMomStep! (ρuf·inv(δt) + BC!); 1: advect! (ρuf as computed)
+    T dt, lr, idt, rtol, ctol, ubc_d;
+};
+
+template <class T>
+__device__ inline T vof_face_flux(const SweepArgs<T> &a, const int q[3], long IF, T dl) {
+    const Grid &g = a.g;
+    int d = a.d;
+    int c[3] = {q[0], q[1], q[2]};
+    if (dl > 0) c[d] -= 1;
+    long IC = IF - (dl > 0 ? g.st[d] : 0);
+    T fc = a.fin[IC];
+    bool ghost = c[d] == 0 || c[d] == g.Ng[d] - 1;
+    if (fullorempty(fc) || (ghost && !g.per[d])) return fc * dl;
+    long ICr = IC;
+    if (ghost) {  // BCVOF! copies n̂, α of the periodic image (VOFutil.jl:44-62)
+        int cc[3] = {c[0], c[1], c[2]};
+        cc[d] = map_scalar(g, d, c[d]);
+        ICr = lin(g, cc);
+    }
+    T n[3];
+    normal_scheme(a.scheme, g, a.fin, ICr, n);
+    T al = getIntercept(g.D, n, fc);
+    T aa = dl > 0 ? al - n[d] * (1 - dl) : al;
+    T nc[3];
+    for (int ii = 0; ii < g.D; ++ii) nc[ii] = n[ii] * (ii == d ? fabs(dl) : T(1));
+    return getVolumeFraction(g.D, nc, aa) * dl;
+}
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_vof_sweep(SweepArgs<T> a) {
+    const Grid &g = a.g;
+    int p[3];
+    bool act = thread_cell(g, a.t, p);
+    FillPart fp;
+    fp.vmax = -1e300;
+    fp.vmin = 1e300;
+    fp.imax = fp.imin = 1L << 60;
+    fp.nan = 0;
+    if (act) {
+        int d = a.d;
+        long I = lin(g, p), sd = g.st[d];
+        const T *u1 = a.u1, *u2 = a.u2;
+        // lower face (IFace = I) and upper face (IFace = I + δd)
+        T dl0 = (a.dt / 2) * (u1[I] + u2[I]);
+        T ff0 = dl0 == T(0) ? T(0) : vof_face_flux(a, p, I, dl0);
+        int pu[3] = {p[0], p[1], p[2]};
+        pu[d] += 1;
+        T dl1 = (a.dt / 2) * (u1[I + sd] + u2[I + sd]);
+        T ff1 = dl1 == T(0) ? T(0) : vof_face_flux(a, pu, I + sd, dl1);
+        // ρuf (advection.jl:127/:135 fᶠ2ρuf, flow.jl:207 rmul! + BC!)
+        bool nper = !g.per[d];
+        T m0 = dl0 == T(0) ? T(0) : T(0) + (dl0 * a.lr + (1 - a.lr) * ff0);
+        if (a.mode == 0) {
+            m0 = m0 * a.idt;
+            if (nper && p[d] == 1) m0 = a.ubc_d;
+        }
+        a.ruf[I] = m0;
+        if (p[d] == g.Ng[d] - 2) {
+            T m1 = dl1 == T(0) ? T(0) : T(0) + (dl1 * a.lr + (1 - a.lr) * ff1);
+            if (a.mode == 0) m1 = nper ? a.ubc_d : m1 * a.idt;
+            a.ruf[I + sd] = m1;
+        }
+        // update (advection.jl:83)
+        T dud = u1[I + sd] - u1[I], du0d = u2[I + sd] - u2[I];
+        T fnew = a.fin[I] + ((ff0 - ff1) + (((T)a.cb[I] * (dud + du0d)) * a.dt) / 2);
+        if (fnew != fnew) fp.nan = 1;
+        else {
+            fp.vmax = (double)fnew;
+            fp.vmin = (double)fnew;
+            fp.imax = fp.imin = I;
+        }
+        // cleanWisp! (VOFutil.jl:127-137) and BCf!
+        T fc = fnew < a.ctol ? T(0) : (fnew > 1 - a.ctol ? T(1) : fnew);
+        write_images<T>(g, a.fout, p, fc, 0, 0, T(0));
+        if (a.fmid) write_images<T>(g, a.fmid, p, (fc + a.fn[I]) / 2, 0, 0, T(0));
+    }
+    FillPart r = block_fill(fp);
+    if (threadIdx.x == 0) a.part[blockIdx.x] = r;
+}
+
+// reportFillError (advection.jl:145-189) on the reduced partials
+template <class T>
+__global__ void __launch_bounds__(NT) k_fill_finalize(Grid g, const FillPart *part, long nblk, const T *u1,
+                                                      const T *u2, double tol, StepState *st) {
+    FillPart acc;
+    acc.vmax = -1e300;
+    acc.vmin = 1e300;
+    acc.imax = acc.imin = 1L << 60;
+    acc.nan = 0;
+    for (long b = threadIdx.x; b < nblk; b += NT) fill_combine(acc, part[b]);
+    FillPart r = block_fill(acc);
+    if (threadIdx.x != 0) return;
+    int s = 0;
+    if (r.nan) s = 1;
+    else {
+        auto divabs = [&](const T *u, long I) {
+            T sdiv = 0;
+            for (int i = 0; i < g.D; ++i) sdiv += u[g.ns * i + I + g.st[i]] - u[g.ns * i + I];
+            return fabs(sdiv);
+        };
+        if (r.vmax - 1 > tol) {
+            T du0 = divabs(u1, r.imax), du = divabs(u2, r.imax);
+            s = (du0 + du > T(10) || (du0 + du) != (du0 + du)) ? 2 : 3;
+        }
+        if (s != 2 && -r.vmin > tol) {
+            T du0 = divabs(u1, r.imin), du = divabs(u2, r.imin);
+            int s2 = (du0 + du > T(10) || (du0 + du) != (du0 + du)) ? 2 : 3;
+            s = s2 == 2 ? 2 : (s ? s : s2);
+        }
+    }
+    if (s == 3) st->nwarn += 1;
+    int old = st->status;
+    int worst = (old == 1 || s == 1) ? 1 : ((old == 2 || s == 2) ? 2 : (old > s ? old : s));
+    if (worst != old && (s == 1 || s == 2 || old == 0)) {
+        st->maxf = r.vmax;
+        st->minf = r.vmin;
+        st->imax = r.imax;
+        st->imin = r.imin;
+    }
+    st->status = worst;
+}
+
+// ---------------------------------------------------------------------------
+// CMOM momentum sweep along d with the SynDRoM blend (flow.jl:20-57, :212-241)
+// ---------------------------------------------------------------------------
+template <class T> struct MomArgs {
+    Grid g;
+    Tiling t;
+    const T *ru_in;
+    T *ru_out;
+    const T *fpre;       // f before this sweep's VOF update (Φ), with ghosts
+    const T *ruf;        // mass flux ×inv(δt), component d (BC! applied on read)
+    const int8_t *cb;
+    const T *u1, *u2;    // velocity component d (dilation)
+    const T *uold;       // uOld (vector)
+    int d, lim;
+    T dt, lr;
+    T ubc[3];
+};
+
+template <class T> __device__ inline T mom_ustar(const MomArgs<T> &a, int i, const int q[3]) {
+    const Grid &g = a.g;
+    if (vel_dirichlet(g, i, q)) return a.ubc[i];
+    long K = lin_vel(g, i, q);
+    return a.ru_in[g.ns * i + K] / linp((a.fpre[K] + a.fpre[K - g.st[i]]) / 2, a.lr);
+}
+template <class T> __device__ inline T mom_ruf(const MomArgs<T> &a, const int q[3]) {
+    const Grid &g = a.g;
+    int d = a.d;
+    if (!g.per[d] && (q[d] == 0 || q[d] == 1 || q[d] == g.Ng[d] - 1)) return a.ubc[d];
+    return a.ruf[lin_scalar(g, q[0], q[1], q[2])];
+}
+template <class T> __device__ inline T mom_drho(const MomArgs<T> &a, int i, const int q[3]) {
+    const Grid &g = a.g;
+    int qq[3];
+    if (!face_map(g, i, q, qq)) return T(1);  // dρ = ones(...) is never written there (cVOF.jl:198)
+    long K = lin(g, qq);
+    return (a.fpre[K] + a.fpre[K - g.st[i]]) / 2;
+}
+template <class T> __device__ inline T mom_dila(const MomArgs<T> &a, const int q[3]) {
+    const Grid &g = a.g;
+    long K = lin_scalar(g, q[0], q[1], q[2]);
+    long s = g.st[a.d];
+    return (linp((T)a.cb[K], a.lr) * ((a.u1[K + s] - a.u1[K]) + (a.u2[K + s] - a.u2[K]))) / 2;
+}
+// ϕq (flow.jl:37-57) for face q of the staggered cell of component i
+template <class T>
+__device__ inline T mom_phiq(const MomArgs<T> &a, int i, const int q[3], T Psi, T uu, T cc, T dd) {
+    int d = a.d;
+    int c[3] = {q[0], q[1], q[2]};
+    if (Psi > 0) c[d] -= 1;
+    T vI = cc, vd = limiter(a.lim, uu, cc, dd);
+    T va = 2 * vI - vd;
+    T mOut = fabs(Psi) * a.dt;
+    T mOld = linp(mom_drho(a, i, c), a.lr);
+    if (mOut > mOld) return Psi * vI;
+    T l2 = fabs(mOut) / mOld, l1 = 1 - l2;
+    T vb = l2 * va + l1 * vd;
+    return (Psi * (vb + vd)) / 2;
+}
+// face flux Φ at face q (q[d] in 1..Ng-1): ϕu, ϕuL, ϕuP, ϕuR
+template <class T> __device__ inline T mom_face(const MomArgs<T> &a, int i, const int qin[3]) {
+    const Grid &g = a.g;
+    int d = a.d, n = g.Ng[d];
+    int q[3] = {qin[0], qin[1], qin[2]};
+    if (g.per[d] && q[d] == n - 1) q[d] = 1;  // flow.jl:241 reuses the lower-boundary flux
+    int qm[3] = {q[0], q[1], q[2]};
+    qm[i] -= 1;
+    T Psi = (mom_ruf(a, q) + mom_ruf(a, qm)) / 2;
+    auto us = [&](int off) {
+        int r[3] = {q[0], q[1], q[2]};
+        r[d] += off;
+        return mom_ustar(a, i, r);
+    };
+    if (q[d] == 1) {
+        if (!g.per[d]) {  // ϕuL
+            T c0 = us(0), cm = us(-1);
+            return Psi > 0 ? mom_phiq(a, i, q, Psi, 2 * cm - c0, cm, c0) : mom_phiq(a, i, q, Psi, us(1), c0, cm);
+        }
+        // ϕuP: the far-upwind value is the periodic image of I-2δ
+        int r[3] = {q[0], q[1], q[2]};
+        r[d] = n - 3;
+        return Psi > 0 ? mom_phiq(a, i, q, Psi, mom_ustar(a, i, r), us(-1), us(0))
+                       : mom_phiq(a, i, q, Psi, us(1), us(0), us(-1));
+    }
+    if (q[d] == n - 1) {  // ϕuR
+        T c0 = us(0), cm = us(-1);
+        return Psi < 0 ? mom_phiq(a, i, q, Psi, 2 * c0 - cm, c0, cm) : mom_phiq(a, i, q, Psi, us(-2), cm, c0);
+    }
+    return Psi > 0 ? mom_phiq(a, i, q, Psi, us(-2), us(-1), us(0)) : mom_phiq(a, i, q, Psi, us(1), us(0), us(-1));
+}
+
+template <class T>
+__global__ void __launch_bounds__(NT) k_momentum(MomArgs<T> a) {
+    const Grid &g = a.g;
+    int p[3];
+    if (!thread_cell(g, a.t, p)) return;
+    long I = lin(g, p);
+    int d = a.d;
+    T dl0 = mom_dila(a, p);
+    for (int i = 0; i < g.D; ++i) {
+        int pu[3] = {p[0], p[1], p[2]};
+        pu[d] += 1;
+        T flo = mom_face(a, i, p), fup = mom_face(a, i, pu);
+        int pm[3] = {p[0], p[1], p[2]};
+        pm[i] -= 1;
+        T r = (T(0) + flo) - fup;
+        r += a.uold[g.ns * i + I] * ((dl0 + mom_dila(a, pm)) / 2);
+        long o = g.ns * i + I;
+        a.ru_out[o] = a.ru_in[o] + a.dt * r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// viscous stress (flow.jl:113-153) fused with the RK update (flow.jl:244-252)
+// and the Poisson coefficients (flow.jl:254-259)
+// ---------------------------------------------------------------------------
+template <class T> struct ViscArgs {
+    Grid g;
+    Tiling t;
+    const T *u;       // velocity the stress is taken from (ghosts valid)
+    const T *fv;      // f for the face fractions
+    T lm, mu, lr;
+    T ubc[3];
+    int has_mu;
+};
+// face fraction fFace[q, c] as left by f2face!/BCv! into n̂ (VOFutil.jl:229-234)
+template <class T> __device__ inline T visc_ff(const ViscArgs<T> &a, int c, const int q[3]) {
+    const Grid &g = a.g;
+    int qq[3];
+    if (!face_map(g, c, q, qq)) return a.ubc[c];  // n̂ there still holds uStar = u_BC (flow.jl:206)
+    long K = lin(g, qq);
+    return (a.fv[K] + a.fv[K - g.st[c]]) / 2;
+}
+template <class T> __device__ inline T viscF(const ViscArgs<T> &a, int i, int j, const int qin[3]) {
+    const Grid &g = a.g;
+    int q[3] = {qin[0], qin[1], qin[2]};
+    if (g.per[j] && q[j] == g.Ng[j] - 1) q[j] = 1;  // flow.jl:153
+    int qj[3] = {q[0], q[1], q[2]}, qi[3] = {q[0], q[1], q[2]};
+    qj[j] -= 1;
+    qi[i] -= 1;
+    T f1 = visc_ff(a, i, qj), f2 = visc_ff(a, i, q), f3 = visc_ff(a, j, qi), f4 = visc_ff(a, j, q);
+    T s = (((f1 + f2) + f3) + f4) / 4;
+    T fr = a.lr < T(1) ? tmin(tmin(tmin(f1, f2), f3), f4) : tmax(tmax(tmax(f1, f2), f3), f4);
+    T mu = a.mu * tmin(linp(s, a.lm), (s > T(0.5) ? T(1) : a.lm / a.lr) * linp(fr, a.lr));
+    long Q = lin(g, q);
+    const T *ui = a.u + g.ns * i, *uj = a.u + g.ns * j;
+    return mu * ((ui[Q] - ui[Q - g.st[j]]) + (uj[Q] - uj[Q - g.st[i]]));
+}
+template <class T> __device__ inline T visc_force(const ViscArgs<T> &a, int i, const int p[3]) {
+    if (!a.has_mu) return T(0);
+    T r = 0;
+    for (int j = 0; j < a.g.D; ++j) {
+        int pu[3] = {p[0], p[1], p[2]};
+        pu[j] += 1;
+        r += -viscF(a, i, j, p);
+        r -= -viscF(a, i, j, pu);
+    }
+    return r;
+}
+
+template <class T> struct UpdArgs {
+    ViscArgs<T> v;
+    const T *u0;      // Flow.u⁰ (for ρu⁰ = u⁰ρ(f_cur), flow.jl:76/:99)
+    const T *fcur;    // cVOF.f
+    const T *fupd;    // f of the update and of the coefficients
+    const T *ru;      // ρu after advection
+    T *uout;
+    T *L;             // μ₀
+    T dt, w, g[3];
+};
+template <class T>
+__global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
+    const Grid &g = a.v.g;
+    int p[3];
+    if (!thread_cell(g, a.v.t, p)) return;
+    long I = lin(g, p);
+    T wT = a.w, ac = T(1) / wT - 1;
+    for (int i = 0; i < g.D; ++i) {
+        long o = g.ns * i + I;
+        T F = visc_force(a.v, i, p);
+        T ru0 = a.u0[o] * linp((a.fcur[I] + a.fcur[I - g.st[i]]) / 2, a.v.lr);
+        T run = ((ac * ru0 + a.ru[o]) + F * a.dt) * wT;
+        T rho = linp((a.fupd[I] + a.fupd[I - g.st[i]]) / 2, a.v.lr);
+        T un = run / rho;
+        un = un + (a.dt * wT) * a.g[i];
+        write_images<T>(g, a.uout + g.ns * i, p, un, 1, i, a.v.ubc[i]);
+        write_images<T>(g, a.L + g.ns * i, p, T(1) / rho, 1, i, T(0));
+    }
+}
+// viscSurfTenρu! alone (stage entry point): Flow.f = visc
+template <class T>
+__global__ void __launch_bounds__(NT) k_visc_force(ViscArgs<T> a, T *F) {
+    const Grid &g = a.g;
+    int p[3];
+    if (!thread_cell(g, a.t, p)) return;
+    long I = lin(g, p);
+    for (int i = 0; i < g.D; ++i) F[g.ns * i + I] = visc_force(a, i, p);
+}
+// updateL! alone (stage entry point)
+template <class T>
+__global__ void __launch_bounds__(NT) k_update_L(Grid g, Tiling t, const T *f, T *L, T lr) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    for (int i = 0; i < g.D; ++i)
+        write_images<T>(g, L + g.ns * i, p, T(1) / linp((f[I] + f[I - g.st[i]]) / 2, lr), 1, i, T(0));
+}
+
+// ---------------------------------------------------------------------------
+// WaterLily MultiLevelPoisson (Poisson.jl / MultiLevelPoisson.jl semantics)
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline T xper(const Grid &g, const T *x, int p0, int p1, int p2) {
+    return x[(long)map_per(g, 0, p0) + g.st[1] * map_per(g, 1, p1) + g.st[2] * map_per(g, 2, p2)];
+}
+// mult(I,L,D,x) with perBC! applied to x on read
+template <class T> __device__ inline T mg_mult(const Grid &g, const int p[3], long I, const T *L, const T *Dg, const T *x) {
+    T sl = 0, su = 0;
+    for (int i = 0; i < g.D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] -= 1;
+        sl += xper(g, x, q[0], q[1], q[2]) * L[g.ns * i + I];
+    }
+    for (int i = 0; i < g.D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] += 1;
+        su += xper(g, x, q[0], q[1], q[2]) * L[g.ns * i + I + g.st[i]];
+    }
+    return (x[I] * Dg[I] + sl) + su;
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_set_diag(Grid g, Tiling t, const T *L, T *Dg, T *iD) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T s = 0;
+    for (int d = 0; d < g.D; ++d) s -= (L[g.ns * d + I] + L[g.ns * d + I + g.st[d]]);
+    Dg[I] = s;
+    iD[I] = (s * s) < 2 * eps_of<T>() ? T(0) : T(1) / s;
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_restrictL(Grid gc, Tiling t, Grid gf, const T *Lf, T *Lc) {
+    int p[3];
+    if (!thread_cell(gc, t, p)) return;
+    for (int ii = 0; ii < gc.D; ++ii) {
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = a < gc.D ? 2 * p[a] - 1 : 0;
+            hi[a] = a < gc.D ? (a == ii ? 2 * p[a] - 1 : 2 * p[a]) : 0;
+        }
+        T s = 0;
+        for (int c2 = lo[2]; c2 <= hi[2]; ++c2)
+            for (int c1 = lo[1]; c1 <= hi[1]; ++c1)
+                for (int c0 = lo[0]; c0 <= hi[0]; ++c0) s += Lf[gf.ns * ii + c0 + gf.st[1] * c1 + gf.st[2] * c2];
+        write_images<T>(gc, Lc + gc.ns * ii, p, (T)(0.5 * (double)s), 1, ii, T(0));
+    }
+}
+// z = div(u); (flow.jl:345)
+template <class T>
+__global__ void __launch_bounds__(NT) k_div(Grid g, Tiling t, const T *u, T *z) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T s = 0;
+    for (int i = 0; i < g.D; ++i) s += u[g.ns * i + I + g.st[i]] - u[g.ns * i + I];
+    z[I] = s;
+}
+template <class T>
+__global__ void k_scale(T *x, long n, T s) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = x[q] * s;
+}
+template <class T>
+__global__ void k_fill(T *x, long n, T v) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = v;
+}
+// residual! part 1: r = z - A x (0 where iD == 0); block sums of r
+template <class T>
+__global__ void __launch_bounds__(NT) k_residual(Grid g, Tiling t, const T *L, const T *Dg, const T *iD, const T *x,
+                                                 const T *z, T *r, double *part) {
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        T rr = iD[I] == T(0) ? T(0) : z[I] - mg_mult(g, p, I, L, Dg, x);
+        r[I] = rr;
+        v = (double)rr;
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// residual! part 2: s = Σr / n; flag when |s| > 2eps
+template <class T>
+__global__ void __launch_bounds__(NT) k_fin_mean(const double *part, long nblk, long ncell, SolveCtl *ctl) {
+    double v = 0;
+    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
+    double s = block_sum(v);
+    if (threadIdx.x == 0) {
+        T m = (T)s / (T)ncell;
+        ctl->s = (double)m;
+        ctl->done = fabs(m) <= 2 * eps_of<T>() ? 1 : 0;
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_sub_mean(Grid g, Tiling t, T *r, const SolveCtl *ctl) {
+    if (ctl->done) return;
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    r[I] = r[I] - (T)ctl->s;
+}
+// L₂ = r·r (block sums)
+template <class T>
+__global__ void __launch_bounds__(NT) k_dot(Grid g, Tiling t, const T *a, const T *b, double *part) {
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        v = (double)a[I] * (double)b[I];
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(NT) k_fin_sum(const double *part, long nblk, double *out) {
+    double v = 0;
+    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
+    double s = block_sum(v);
+    if (threadIdx.x == 0) *out = s;
+}
+// Jacobi!: ϵ = r·iD
+template <class T>
+__global__ void __launch_bounds__(NT) k_jacobi_eps(Grid g, Tiling t, const T *r, const T *iD, T *e) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    e[I] = r[I] * iD[I];
+}
+// increment!: r -= A ϵ; x += ϵ
+template <class T>
+__global__ void __launch_bounds__(NT) k_increment(Grid g, Tiling t, const T *L, const T *Dg, const T *e, T *r, T *x) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    r[I] = r[I] - mg_mult(g, p, I, L, Dg, e);
+    x[I] = x[I] + e[I];
+}
+// restrict!: coarse r = Σ children
+template <class T>
+__global__ void __launch_bounds__(NT) k_restrict(Grid gc, Tiling t, Grid gf, const T *rf, T *rc, T *xc) {
+    int p[3];
+    if (!thread_cell(gc, t, p)) return;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = a < gc.D ? 2 * p[a] - 1 : 0;
+        hi[a] = a < gc.D ? 2 * p[a] : 0;
+    }
+    T s = 0;
+    for (int c2 = lo[2]; c2 <= hi[2]; ++c2)
+        for (int c1 = lo[1]; c1 <= hi[1]; ++c1)
+            for (int c0 = lo[0]; c0 <= hi[0]; ++c0) s += rf[c0 + gf.st[1] * c1 + gf.st[2] * c2];
+    long I = lin(gc, p);
+    rc[I] = s;
+    xc[I] = 0;
+}
+// prolongate!: fine ϵ = coarse x[down(I)]
+template <class T>
+__global__ void __launch_bounds__(NT) k_prolong(Grid gf, Tiling t, Grid gc, const T *xc, T *ef) {
+    int p[3];
+    if (!thread_cell(gf, t, p)) return;
+    int q[3];
+    for (int a = 0; a < 3; ++a) q[a] = a < gf.D ? (p[a] + 3) / 2 - 1 : 0;
+    ef[lin(gf, p)] = xc[lin(gc, q)];
+}
+// pcg! (it = 6): init z = ϵ = r·iD, ρ = r·z
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_init(Grid g, Tiling t, const T *r, const T *iD, T *z, T *e, double *part) {
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        T zz = r[I] * iD[I];
+        z[I] = zz;
+        e[I] = zz;
+        v = (double)r[I] * (double)zz;
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_fin_rho0(const double *part, long nblk, SolveCtl *ctl) {
+    double v = 0;
+    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
+    double s = block_sum(v);
+    if (threadIdx.x == 0) {
+        T rho = (T)s;
+        ctl->rho = (double)rho;
+        ctl->done = fabs(rho) < 10 * eps_of<T>() ? 1 : 0;
+        ctl->stop = 0;
+        ctl->it = 0;
+    }
+}
+// z = A ϵ; partial z·ϵ
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_Ae(Grid g, Tiling t, const T *L, const T *Dg, const T *e, T *z, double *part,
+                                               const SolveCtl *ctl) {
+    if (ctl->done) return;
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        T zz = mg_mult(g, p, I, L, Dg, e);
+        z[I] = zz;
+        v = (double)zz * (double)e[I];
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_fin_alpha(const double *part, long nblk, SolveCtl *ctl, int it) {
+    if (ctl->done) return;
+    double v = 0;
+    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
+    double s = block_sum(v);
+    if (threadIdx.x == 0) {
+        T alpha = (T)ctl->rho / (T)s;
+        ctl->alpha = (double)alpha;
+        ctl->it += 1;
+        ctl->stop = (ctl->it == it || fabs((double)alpha) < 1e-2) ? 1 : 0;
+    }
+}
+// x += αϵ; r -= αz; then z = r·iD and partial r·z
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_update(Grid g, Tiling t, const T *iD, const T *e, T *z, T *r, T *x,
+                                                   double *part, const SolveCtl *ctl) {
+    if (ctl->done) return;
+    T alpha = (T)ctl->alpha;
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        x[I] += alpha * e[I];
+        T rr = r[I] - alpha * z[I];
+        r[I] = rr;
+        T zz = rr * iD[I];
+        z[I] = zz;
+        v = (double)rr * (double)zz;
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_fin_beta(const double *part, long nblk, SolveCtl *ctl) {
+    if (ctl->done) return;
+    double v = 0;
+    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
+    double s = block_sum(v);
+    if (threadIdx.x == 0) {
+        if (ctl->stop) { ctl->done = 1; return; }
+        T rho2 = (T)s;
+        if (fabs(rho2) < 10 * eps_of<T>()) { ctl->done = 1; return; }
+        T beta = rho2 / (T)ctl->rho;
+        ctl->beta = (double)beta;
+        ctl->rho = (double)rho2;
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_eps(Grid g, Tiling t, const T *z, T *e, const SolveCtl *ctl) {
+    if (ctl->done) return;
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    e[I] = (T)ctl->beta * e[I] + z[I];
+}
+// perBC!(x)
+template <class T>
+__global__ void __launch_bounds__(NT) k_perbc(Grid g, Tiling t, T *x) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    bool any = false;
+    for (int a = 0; a < g.D; ++a) any |= g.per[a] && (p[a] == 1 || p[a] == g.Ng[a] - 2);
+    if (!any) return;
+    // periodic images only (non-periodic ghosts keep their values)
+    int alt[3][2], na[3];
+    for (int a = 0; a < 3; ++a) {
+        na[a] = 1;
+        alt[a][0] = p[a];
+        if (a < g.D && g.per[a]) {
+            if (p[a] == 1) { alt[a][1] = g.Ng[a] - 1; na[a] = 2; }
+            else if (p[a] == g.Ng[a] - 2) { alt[a][1] = 0; na[a] = 2; }
+        }
+    }
+    T v = x[lin(g, p)];
+    for (int i2 = 0; i2 < na[2]; ++i2)
+        for (int i1 = 0; i1 < na[1]; ++i1)
+            for (int i0 = 0; i0 < na[0]; ++i0)
+                x[(long)alt[0][i0] + g.st[1] * alt[1][i1] + g.st[2] * alt[2][i2]] = v;
+}
+// myproject!: u -= L ∂x ; BC!(u)  (flow.jl:331-333, :82)
+template <class T>
+__global__ void __launch_bounds__(NT) k_project(Grid g, Tiling t, T *u, const T *L, const T *x, T ubc0, T ubc1, T ubc2) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T ub[3] = {ubc0, ubc1, ubc2};
+    for (int i = 0; i < g.D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] -= 1;
+        T dx = x[I] - xper(g, x, q[0], q[1], q[2]);
+        T un = u[g.ns * i + I] - L[g.ns * i + I] * dx;
+        write_images<T>(g, u + g.ns * i, p, un, 1, i, ub[i]);
+    }
+}
+// MPCFL (flow.jl:262-281): block maxima of flux_out and maxTotalFlux
+template <class T>
+__global__ void __launch_bounds__(NT) k_cfl(Grid g, Tiling t, const T *u, double *part) {
+    int p[3];
+    double a = 0, b = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        double acc = 0;
+        T s2 = 0;
+        for (int d = 0; d < g.D; ++d) {
+            double up = (double)u[g.ns * d + I + g.st[d]], lo = -(double)u[g.ns * d + I];
+            acc += (up > 0.0 ? up : 0.0) + (lo > 0.0 ? lo : 0.0);
+            s2 += tmax(fabs(u[g.ns * d + I]), fabs(u[g.ns * d + I + g.st[d]]));
+        }
+        a = (double)(T)acc;
+        b = (double)s2;
+    }
+    double ma = block_max(a);
+    __syncthreads();
+    double mb = block_max(b);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ma;
+        part[2 * blockIdx.x + 1] = mb;
+    }
+}
+__global__ void __launch_bounds__(NT) k_fin_max2(const double *part, long nblk, double *out) {
+    double a = 0, b = 0;
+    for (long q = threadIdx.x; q < nblk; q += NT) {
+        a = part[2 * q] > a ? part[2 * q] : a;
+        b = part[2 * q + 1] > b ? part[2 * q + 1] : b;
+    }
+    double ma = block_max(a);
+    __syncthreads();
+    double mb = block_max(b);
+    if (threadIdx.x == 0) { out[0] = ma; out[1] = mb; }
+}
+// f⁰ = (f⁰ + f)/2 over every cell (flow.jl:74)
+template <class T>
+__global__ void k_avg(T *a, const T *b, long n) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = (a[q] + b[q]) / 2;
+}
+// applyVOF! from SDF samples (VOFutil.jl:203-225) + cleanWisp! + BCf!
+template <class T>
+__global__ void __launch_bounds__(NT) k_apply_vof(Grid g, Tiling t, const T *sdf, T *f, T *al, T *nh, T ctol) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    long q = (long)(p[0] - 1) + (long)(g.Ng[0] - 2) * ((g.D > 1 ? p[1] - 1 : 0) + (long)(g.D > 1 ? g.Ng[1] - 2 : 1) * (g.D > 2 ? p[2] - 1 : 0));
+    const T *s = sdf + q * (1 + 2 * g.D);
+    T sumN = 0, sumN2 = 0, n[3];
+    for (int d = 0; d < g.D; ++d) {
+        T dd = s[2 * d + 1] - s[2 * d + 2];
+        nh[g.ns * d + I] = dd;
+        n[d] = dd;
+        sumN += dd;
+        sumN2 += dd * dd;
+    }
+    T a = sumN / 2 - sqrt(sumN2) * s[0];
+    al[I] = a;
+    T fv = getVolumeFraction(g.D, n, a);
+    fv = fv < ctol ? T(0) : (fv > 1 - ctol ? T(1) : fv);
+    write_images<T>(g, f, p, fv, 0, 0, T(0));
+}
+// reconstructInterface! materialised (stage entry point; BCVOF! on periodic sides)
+template <class T>
+__global__ void __launch_bounds__(NT) k_reconstruct(Grid g, Tiling t, const T *f, T *al, T *nh, int scheme) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T n[3] = {0, 0, 0}, a = 0;
+    if (!fullorempty(f[I])) {
+        normal_scheme(scheme, g, f, I, n);
+        a = getIntercept(g.D, n, f[I]);
+    }
+    // periodic images only (non-periodic ghosts of n̂/α are zero)
+    int alt[3][2], na[3];
+    for (int ax = 0; ax < 3; ++ax) {
+        na[ax] = 1;
+        alt[ax][0] = p[ax];
+        if (ax < g.D && g.per[ax]) {
+            if (p[ax] == 1) { alt[ax][1] = g.Ng[ax] - 1; na[ax] = 2; }
+            else if (p[ax] == g.Ng[ax] - 2) { alt[ax][1] = 0; na[ax] = 2; }
+        }
+    }
+    for (int i2 = 0; i2 < na[2]; ++i2)
+        for (int i1 = 0; i1 < na[1]; ++i1)
+            for (int i0 = 0; i0 < na[0]; ++i0) {
+                long J = (long)alt[0][i0] + g.st[1] * alt[1][i1] + g.st[2] * alt[2][i2];
+                al[J] = a;
+                for (int d = 0; d < g.D; ++d) nh[g.ns * d + J] = n[d];
+            }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 iterative refinement around the FP32 V-cycle (REFINED mode)
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline double mult64(const Grid &g, const int p[3], long I, const T *L, const double *x) {
+    double dg = 0, sl = 0, su = 0;
+    for (int i = 0; i < g.D; ++i) dg -= ((double)L[g.ns * i + I] + (double)L[g.ns * i + I + g.st[i]]);
+    for (int i = 0; i < g.D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] -= 1;
+        sl += x[(long)map_per(g, 0, q[0]) + g.st[1] * map_per(g, 1, q[1]) + g.st[2] * map_per(g, 2, q[2])] *
+              (double)L[g.ns * i + I];
+    }
+    for (int i = 0; i < g.D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] += 1;
+        su += x[(long)map_per(g, 0, q[0]) + g.st[1] * map_per(g, 1, q[1]) + g.st[2] * map_per(g, 2, q[2])] *
+              (double)L[g.ns * i + I + g.st[i]];
+    }
+    return (x[I] * dg + sl) + su;
+}
+// z64 = div u (double); x64 = p·dt; block sums of z
+template <class T>
+__global__ void __launch_bounds__(NT) k_ir_init(Grid g, Tiling t, const T *u, const T *pT, double dtw, double *z,
+                                                double *x, double *part) {
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        double s = 0;
+        for (int i = 0; i < g.D; ++i) s += (double)u[g.ns * i + I + g.st[i]] - (double)u[g.ns * i + I];
+        z[I] = s;
+        x[I] = (double)pT[I] * dtw;
+        v = s;
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// a -= mean (mean = *sum / n); block sums of a²
+__global__ void __launch_bounds__(NT) k_ir_center(Grid g, Tiling t, double *a, const double *sum, long ncell,
+                                                  double *part) {
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        double m = *sum / (double)ncell;
+        double q = a[I] - m;
+        a[I] = q;
+        v = q * q;
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// r64 = z - A x (0 where iD == 0); block sums of r
+template <class T>
+__global__ void __launch_bounds__(NT) k_ir_resid(Grid g, Tiling t, const T *L, const T *iD, const double *z,
+                                                 const double *x, double *r, double *part) {
+    int p[3];
+    double v = 0;
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        double rr = iD[I] == T(0) ? 0.0 : z[I] - mult64(g, p, I, L, x);
+        r[I] = rr;
+        v = rr;
+    }
+    double s = block_sum(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// T-precision residual scaled to unit RMS; x_T = 0
+template <class T>
+__global__ void __launch_bounds__(NT) k_ir_load(Grid g, Tiling t, const double *r, const double *nrm2, long ncell,
+                                                T *rT, T *xT) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    double sc = sqrt(*nrm2) / sqrt((double)ncell);
+    rT[I] = (T)(r[I] / sc);
+    xT[I] = T(0);
+}
+// x64 += sc·x_T  (periodic ghosts of x_T are refreshed by the caller)
+template <class T>
+__global__ void __launch_bounds__(NT) k_ir_accum(Grid g, Tiling t, const T *xT, const double *nrm2, long ncell,
+                                                 double *x) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    double sc = sqrt(*nrm2) / sqrt((double)ncell);
+    x[I] += sc * (double)xT[I];
+}
+// u -= L ∂x64 ; p = x64/dt ; BC!(u)
+template <class T>
+__global__ void __launch_bounds__(NT) k_ir_project(Grid g, Tiling t, T *u, const T *L, const double *x, T *pT,
+                                                   double dtw, T ubc0, T ubc1, T ubc2) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T ub[3] = {ubc0, ubc1, ubc2};
+    for (int i = 0; i < g.D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] -= 1;
+        double xm = x[(long)map_per(g, 0, q[0]) + g.st[1] * map_per(g, 1, q[1]) + g.st[2] * map_per(g, 2, q[2])];
+        T un = (T)((double)u[g.ns * i + I] - (double)L[g.ns * i + I] * (x[I] - xm));
+        write_images<T>(g, u + g.ns * i, p, un, 1, i, ub[i]);
+    }
+    pT[I] = (T)(x[I] / dtw);
+}
+
+}  // namespace ia

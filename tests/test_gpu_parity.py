@@ -1,0 +1,362 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on
+identical inputs, stage by stage and for whole steps.
+
+Bars (stated per assertion):
+- FP64: every field within 1e-12 (VOF) / 1e-10 (velocity, pressure after the
+  iterative solver, whose dot products are summed in a different order);
+- FP32: VOF within 1e-5 of the FP32 oracle (same operation order, no FMA
+  contraction on either side; the transcendental functions of the PLIC
+  inverse may differ by an ulp), velocity/pressure as stated per test.
+Integer/byte fields (c̄) are bit-exact.
+"""
+import numpy as np
+import pytest
+
+from __graft_entry__ import load_package
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ia = load_package()
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def maxdiff(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max())
+
+
+def tgv_droplet(N=16, T=np.float64, λμ=0.1, λρ=0.1, **kw):
+    """helper.jl:6-13 (TGV droplet, periodic)."""
+    κ = 2 * np.pi / N
+    R = N / 4
+    u0 = lambda i, x: (-np.sin(x[0] * κ) * np.cos(x[1] * κ)) if i == 1 else (np.cos(x[0] * κ) * np.sin(x[1] * κ))
+    sdf = lambda x: np.sqrt((x[0] - N / 2) ** 2 + (x[1] - N / 2) ** 2) - R
+    return dict(dims=(N, N), u_BC=(0.0, 0.0), L=float(N), kw=dict(T=T, λμ=λμ, λρ=λρ, InterfaceSDF=sdf, u0=u0,
+                                                                   perdir=(1, 2), **kw))
+
+
+def quiescent(N=8, T=np.float64):
+    """helper.jl:17-18."""
+    sdf = lambda x: np.sqrt((x[0] - N / 2) ** 2 + (x[1] - N / 2) ** 2) - N / 4
+    return dict(dims=(N, N), u_BC=(0.0, 0.0), L=float(N), kw=dict(T=T, InterfaceSDF=sdf, perdir=(1, 2)))
+
+
+def bubble2d(N=64, T=np.float32, ν=None, g0=None):
+    """Config-2 shape at a small size: free surface with a cosine bump and one bubble, slip walls."""
+    L = float(N)
+    ν = L / 1e4 if ν is None else ν
+    g0 = 1.0 / L if g0 is None else g0
+
+    def sdf(x):
+        h = 0.5 * L + 0.05 * L * np.cos(2 * np.pi * x[0] / L)
+        return np.maximum(x[1] - h, 0.1 * L - np.sqrt((x[0] - 0.5 * L) ** 2 + (x[1] - 0.25 * L) ** 2))
+
+    return dict(dims=(N, N), u_BC=(0.0, 0.0), L=L, kw=dict(T=T, λρ=1e-3, λμ=1e-2, ν=ν, g=(0.0, -g0),
+                                                          InterfaceSDF=sdf))
+
+
+def bubbles3d(N=32, T=np.float32, seed=3, nb=4):
+    L = float(N)
+    rng = np.random.default_rng(seed)
+    cs = [(rng.uniform(0.25, 0.75) * L, rng.uniform(0.25, 0.75) * L, rng.uniform(0.2, 0.45) * L,
+           rng.uniform(0.06, 0.1) * L) for _ in range(nb)]
+
+    def sdf(x):
+        s = x[2] - 0.7 * L
+        for (cx, cy, cz, r) in cs:
+            s = np.maximum(s, r - np.sqrt((x[0] - cx) ** 2 + (x[1] - cy) ** 2 + (x[2] - cz) ** 2))
+        return s
+
+    return dict(dims=(N, N, N), u_BC=(0.0, 0.0, 0.0), L=L, kw=dict(T=T, λρ=1e-3, λμ=1e-2, ν=L / 1e4,
+                                                                     g=(0.0, 0.0, -1.0 / L), InterfaceSDF=sdf))
+
+
+def pair(case, **extra):
+    kw = dict(case["kw"])
+    kw.update(extra)
+    okw = {k: v for k, v in kw.items() if k not in ("poisson", "poisson_rtol", "poisson_max_outer")}
+    gsim = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **kw)
+    osim = O.OracleSim(case["dims"], case["u_BC"], case["L"], **okw)
+    return gsim, osim
+
+
+# --------------------------------------------------------------------------- construction
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("D", [2, 3])
+def test_apply_vof_and_cfl(T, D):
+    say("applyVOF!+MPCFL", T.__name__, f"{D}D")
+    case = bubble2d(32, T) if D == 2 else bubbles3d(16, T)
+    g, o = pair(case)
+    tol = 1e-12 if T == np.float64 else 1e-6
+    assert maxdiff(g.intf.f, o.f) <= tol
+    assert (g.intf.f0 == g.intf.f).all()
+    assert g.flow.Δt[-1] == o.Δt[-1]
+    g.close()
+
+
+# --------------------------------------------------------------------------- the VOF sweep
+def _swirl(case, T, amp=0.6, seed=0):
+    """A discretely divergence-free face field (corner streamfunction in the x-y plane)."""
+    dims = case["dims"]
+    D = len(dims)
+    Ng = [n + 2 for n in dims]
+    x = np.arange(Ng[0]) - 1.0
+    y = np.arange(Ng[1]) - 1.0
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    psi = amp * np.sin(np.pi * X / dims[0]) ** 2 * np.sin(np.pi * Y / dims[1]) ** 2 * dims[0] / np.pi
+    u = np.zeros(tuple(Ng) + (D,))
+    ux = np.zeros((Ng[0], Ng[1]))
+    uy = np.zeros((Ng[0], Ng[1]))
+    ux[:, :-1] = -(psi[:, 1:] - psi[:, :-1])
+    uy[:-1, :] = psi[1:, :] - psi[:-1, :]
+    if D == 2:
+        u[..., 0], u[..., 1] = ux, uy
+    else:
+        zz = 1.0 + 0.3 * np.cos(np.pi * (np.arange(Ng[2]) - 0.5) / dims[2])
+        u[..., 0] = ux[..., None] * zz
+        u[..., 1] = uy[..., None] * zz
+        u[..., 2] = 0.05 * np.sin(2 * np.pi * (X[..., None] + 0.5) / dims[0]) * np.ones(Ng[2])
+    return np.asfortranarray(u, dtype=T)
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("per", [False, True])
+def test_vof_sweep(T, D, per):
+    say("advectVOF1d!", T.__name__, f"{D}D", "periodic" if per else "walls")
+    case = bubble2d(32, T) if D == 2 else bubbles3d(16, T)
+    if per:
+        case["kw"]["perdir"] = tuple(range(1, D + 1))
+    g, o = pair(case)
+    u = _swirl(case, T)
+    u0 = np.asfortranarray(u * T(0.9))
+    g.flow.u = u
+    g.flow.u0 = u0
+    o.set(O.U, u)
+    o.set(O.U0, u0)
+    tol = 1e-12 if T == np.float64 else 1e-5
+    for d in range(1, D + 1):
+        rc = g._call("ia_vof_sweep", d, 0.5, None)
+        ro = o.lib.oc_vof_sweep(o.ctx, d, 0.5)
+        assert rc == ro
+        assert maxdiff(g.intf.f, o.f) <= tol, d
+        assert (g.intf.cbar == o.get(O.CBAR)).all()
+        rg, ro_ = g.intf.rhouf[..., d - 1], o.get(O.RHOUF)[..., d - 1]
+        inner = tuple(slice(1, None if a == d - 1 else -1) for a in range(D))
+        assert maxdiff(rg[inner], ro_[inner]) <= tol * 10
+    g.close()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_advect_single_vortex(T):
+    """advect! (pure cVOF, the config-1 call) for 24 steps on a 64² periodic grid."""
+    say("advect! single vortex 64^2", T.__name__)
+    N = 64
+    sdf = lambda x: np.sqrt((x[0] - 0.5 * N) ** 2 + (x[1] - 0.75 * N) ** 2) - 0.15 * N
+    case = dict(dims=(N, N), u_BC=(0.0, 0.0), L=float(N), kw=dict(T=T, InterfaceSDF=sdf, perdir=(1, 2)))
+    g, o = pair(case)
+    u = _swirl(case, T, amp=1.0)
+    g.flow.u = u
+    g.flow.u0 = u
+    o.set(O.U, u)
+    o.set(O.U0, u)
+    V0 = O.inside(o.f).astype(np.float64).sum()
+    for n in range(1, 25):
+        g._call("ia_advect", 0.25, n, None)
+        o.advect(0.25, n)
+    fg = g.intf.f
+    tol = 1e-12 if T == np.float64 else 1e-5
+    assert maxdiff(fg, o.f) <= tol
+    assert fg.min() >= 0 and fg.max() <= 1
+    assert abs(O.inside(fg).astype(np.float64).sum() - V0) <= (1e-12 if T == np.float64 else 1e-5) * V0
+    g.close()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("D", [2, 3])
+def test_reconstruct(T, D):
+    say("reconstructInterface!", T.__name__, f"{D}D")
+    case = bubble2d(32, T) if D == 2 else bubbles3d(16, T)
+    case["kw"]["perdir"] = (1,)
+    g, o = pair(case)
+    g._call("ia_reconstruct")
+    o.lib.oc_reconstruct(o.ctx)
+    tol = 1e-12 if T == np.float64 else 2e-5
+    assert maxdiff(O.inside(g.intf.alpha), O.inside(o.get(O.ALPHA))) <= tol
+    assert maxdiff(O.inside(g.intf.nhat, D), O.inside(o.get(O.NHAT), D)) <= tol
+    g.close()
+
+
+# --------------------------------------------------------------------------- viscosity, coefficients, solve
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("D", [2, 3])
+def test_visc_and_coefficients(T, D):
+    say("visc!/updateL!/update!", T.__name__, f"{D}D")
+    case = bubble2d(32, T, ν=0.05) if D == 2 else bubbles3d(16, T)
+    g, o = pair(case)
+    u = _swirl(case, T)
+    g.flow.u = u
+    o.set(O.U, u)
+    g._call("ia_visc", 0)
+    o.lib.oc_visc(o.ctx, 0)
+    Fg, Fo = O.inside(g.flow.force, D), O.inside(o.get(O.FORCE), D)
+    scale = max(1e-30, np.abs(Fo).max())
+    assert maxdiff(Fg, Fo) <= (1e-12 if T == np.float64 else 2e-5) * scale
+    g._call("ia_update_L", 0)
+    o.lib.oc_update_L(o.ctx, 0)
+    assert maxdiff(g.flow.mu0, o.get(O.MU0)) <= (1e-12 if T == np.float64 else 1e-6) * 1000
+    g.close()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("per", [(), (1,)])
+def test_project_reference(T, per):
+    say("myproject! (WaterLily solver!)", T.__name__, per)
+    case = bubble2d(32, T)
+    case["kw"]["perdir"] = per
+    g, o = pair(case)
+    rng = np.random.default_rng(7)
+    u = np.zeros(g.Ng + (2,), dtype=T, order="F")
+    u[1:-1, 1:-1] = rng.standard_normal((32, 32, 2)) * 0.1
+    g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
+    o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
+    for s in ("ia_update_L",):
+        g._call(s, 0)
+    g._call("ia_mg_update")
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    st = ia.IASolveStats()
+    g._call("ia_project", 0.5, ia.C.byref(st))
+    it, r2 = o.project(0.5)
+    assert abs(st.iters - it) <= 1
+    tol = 1e-9 if T == np.float64 else 2e-3
+    pg, po = O.inside(g.flow.p), O.inside(o.p)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= tol * max(1.0, np.abs(po).max())
+    assert maxdiff(g.flow.u, o.u) <= tol
+    g.close()
+
+
+def test_project_refined_reaches_1e6():
+    say("refined FP64/FP32 solve, 64^2")
+    case = bubble2d(64, np.float32)
+    g, o = pair(case, poisson="refined", poisson_rtol=1e-6, poisson_max_outer=100)
+    rng = np.random.default_rng(8)
+    u = np.zeros(g.Ng + (2,), dtype=np.float32, order="F")
+    u[1:-1, 1:-1] = rng.standard_normal((64, 64, 2))
+    g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
+    o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
+    g._call("ia_update_L", 0)
+    g._call("ia_mg_update")
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    st = ia.IASolveStats()
+    g._call("ia_project", 1.0, ia.C.byref(st))
+    outer, rel = o.project_refined(1.0, 1e-6, 100)
+    assert st.rel <= 1e-6 and rel <= 1e-6
+    assert abs(st.iters - outer) <= 2
+    pg, po = O.inside(g.flow.p).astype(np.float64), O.inside(o.p).astype(np.float64)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= 1e-4 * np.abs(po).max()
+    g.close()
+
+
+# --------------------------------------------------------------------------- whole steps
+def test_quiescent_step_is_noop():
+    """maintests.jl:211-220 on the GPU path."""
+    say("quiescent droplet 8^2 FP64, sim_step!")
+    case = quiescent()
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])
+    f0 = g.intf.f
+    ia.sim_step_(g)
+    assert (g.flow.u == 0).all()
+    assert (g.intf.f == f0).all()
+    assert np.isfinite(g.flow.p).all()
+    g.close()
+
+
+def test_tgv_droplet_volume():
+    """maintests.jl:222-234 on the GPU path."""
+    say("TGV droplet 16^2 FP64, 4 steps")
+    case = tgv_droplet()
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])
+    ia.sim_step_(g)
+    V0 = ia.inside(g.intf.f).sum()
+    for _ in range(3):
+        ia.sim_step_(g)
+    assert np.isfinite(g.flow.u).all() and np.isfinite(g.intf.f).all()
+    assert abs(ia.inside(g.intf.f).sum() - V0) <= 1e-4 * abs(V0)
+    g.close()
+
+
+def test_tgv_droplet_matches_oracle():
+    say("TGV droplet 16^2 FP64 vs oracle, 4 steps")
+    g, o = pair(tgv_droplet())
+    for n in range(4):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= 1e-11, n
+        assert maxdiff(g.flow.u, o.u) <= 1e-9, n
+        assert abs(g.flow.Δt[-1] - o.Δt[-1]) <= 1e-12
+    pg, po = O.inside(g.flow.p), O.inside(o.p)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= 1e-8 * max(1, np.abs(po).max())
+    g.close()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_bubble2d_steps_match_oracle(T):
+    say("free surface + bubble 64^2 vs oracle, 3 steps", T.__name__)
+    g, o = pair(bubble2d(64, T))
+    for n in range(3):
+        ia.sim_step_(g)
+        o.sim_step()
+        ftol = 1e-11 if T == np.float64 else 1e-5
+        assert maxdiff(g.intf.f, o.f) <= ftol, n
+        umax = max(1e-12, np.abs(o.u).max())
+        assert maxdiff(g.flow.u, o.u) <= (1e-8 if T == np.float64 else 2e-3) * umax, n
+    g.close()
+
+
+def test_bubbles3d_step_matches_oracle():
+    say("3D pool + 4 bubbles 32^3 FP32 vs oracle, 2 steps")
+    g, o = pair(bubbles3d(32, np.float32))
+    for n in range(2):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= 1e-5, n
+        umax = max(1e-12, np.abs(o.u).max())
+        assert maxdiff(g.flow.u, o.u) <= 2e-3 * umax, n
+    g.close()
+
+
+def test_uniform_flow_stays_uniform():
+    """CMOM consistency on the GPU: uniform u across a 1000:1 droplet stays uniform."""
+    say("uniform flow 16^2 FP64")
+    N = 16
+    Uv = (0.37, -0.21)
+    sdf = lambda x: np.sqrt((x[0] - 8.2) ** 2 + (x[1] - 7.7) ** 2) - 4.1
+    g = ia.TwoPhaseSimulation((N, N), (0.0, 0.0), T=np.float64, InterfaceSDF=sdf,
+                              u0=lambda i, x: np.full_like(x[0], Uv[i - 1]), perdir=(1, 2), λρ=1e-3)
+    for _ in range(3):
+        ia.sim_step_(g)
+    u = ia.inside(g.flow.u, 2)
+    for i in range(2):
+        assert np.abs(u[..., i] - Uv[i]).max() < 1e-12
+    g.close()
+
+
+def test_nan_is_reported():
+    """reportFillError: a NaN in f raises (advection.jl:148)."""
+    say("NaN injection 16^2")
+    case = tgv_droplet()
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])
+    f = g.intf.f
+    f[5, 5] = np.nan
+    g.intf.f = f
+    with pytest.raises(ia.IAError) as e:
+        ia.sim_step_(g)
+    assert e.value.code == 1
+    g.close()
