@@ -1,0 +1,220 @@
+"""Benchmark: Mcell-steps/s of the two-phase step at 256^3 on one MI355X.
+
+Workload (the 3D two-phase configuration): 256^3 cells, FP32 stencils, slip
+walls, gravity -z, density ratio 1000 (λρ = 1e-3), viscosity ratio 100
+(λμ = 1e-2), a liquid pool z < 0.7L holding 32 spherical bubbles with radii
+U(0.03, 0.08)L at non-overlapping centres (numpy PCG64 seed 3, >= 2-cell
+clearance from walls, surface and each other), L = 256 cells, U = 1,
+g0 = U^2/L, ν = UL/1e4; the pressure solve is FP32 MultiLevelPoisson
+V-cycles inside FP64 iterative refinement to ||z-Ax||/||z|| <= 1e-6.
+
+A step is one full MPFMomStep! (predictor + corrector: 6 VOF sweeps, 6
+momentum sweeps, 2 viscous/RK updates, 2 projections, CFL).  Inputs are
+resident in HBM before the timed region; the timed region brackets K steps
+with device synchronisations.
+
+Prints one JSON line (see the driver contract): value = whole-job
+Mcell-steps/s, roofline of the dominant kernel (algorithmic bytes per launch
+/ average launch time from HIP events recorded around that kernel during the
+timed region), and the CPU oracle timed on the host cores over a bounded
+sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+from __graft_entry__ import load_package  # noqa: E402
+
+METRIC = "Mcell-steps/s at 256^3 two-phase; Poisson time-to-1e-6 learned vs plain MG"
+HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
+
+
+def bubbles(L, nb, seed, rmin, rmax, zmax, clearance=2.0):
+    """Rejection-sampled non-overlapping bubbles inside the pool z < zmax."""
+    rng = np.random.default_rng(seed)
+    out = []
+    tries = 0
+    while len(out) < nb:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("could not place bubbles")
+        r = rng.uniform(rmin, rmax) * L
+        c = rng.uniform(0, L, 3)
+        lo, hi = r + clearance, L - r - clearance
+        if not (lo <= c[0] <= hi and lo <= c[1] <= hi and lo <= c[2] <= zmax * L - r - clearance):
+            continue
+        if any(np.linalg.norm(c - np.array(b[:3])) < r + b[3] + clearance for b in out):
+            continue
+        out.append((c[0], c[1], c[2], r))
+    return out
+
+
+def pool_sdf_samples(N, bl, zmax, T=np.float32):
+    """applyVOF! samples (centre, ±Δx per axis) of the SDF
+    φ = max(z - zmax·L, max_b(r_b - |x - c_b|)) for every interior cell.
+
+    The SDF is evaluated exactly within 3 cells of every bubble; elsewhere the
+    plane term alone is used.  Where they differ both are below -3, and the
+    PLIC formula then gives f = 1 exactly either way, so f is identical to the
+    full evaluation."""
+    L = float(N)
+    dx = T(0.01)
+    D = 3
+    z = (np.arange(N, dtype=T) + T(0.5))
+    S = np.empty((N, N, N, 1 + 2 * D), dtype=T, order="F")
+    zc = z[None, None, :] - T(zmax * L)
+    S[..., 0] = zc
+    S[..., 1] = zc
+    S[..., 2] = zc
+    S[..., 3] = zc
+    S[..., 4] = zc
+    S[..., 5] = (z + dx).astype(T)[None, None, :] - T(zmax * L)
+    S[..., 6] = (z - dx).astype(T)[None, None, :] - T(zmax * L)
+    for (cx, cy, cz, r) in bl:
+        lo = [max(0, int(np.floor(c - r - 4))) for c in (cx, cy, cz)]
+        hi = [min(N, int(np.ceil(c + r + 4))) for c in (cx, cy, cz)]
+        ax = [np.arange(lo[a], hi[a], dtype=T) + T(0.5) for a in range(3)]
+        X = np.meshgrid(*ax, indexing="ij")
+        cc = (T(cx), T(cy), T(cz))
+        offs = [(0, 0, 0), (dx, 0, 0), (-dx, 0, 0), (0, dx, 0), (0, -dx, 0), (0, 0, dx), (0, 0, -dx)]
+        blk = S[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2], :]
+        for q, o in enumerate(offs):
+            Y = [(X[a] + T(o[a])).astype(T) for a in range(3)]
+            d = np.sqrt((Y[0] - cc[0]) ** 2 + (Y[1] - cc[1]) ** 2 + (Y[2] - cc[2]) ** 2)
+            blk[..., q] = np.maximum(blk[..., q], (T(r) - d).astype(T))
+    return np.ascontiguousarray(S.reshape(N * N * N, 1 + 2 * D, order="F"))
+
+
+def make_case(N, poisson, T=np.float32, nb=32, seed=3):
+    L = float(N)
+    bl = bubbles(N, nb, seed, 0.03, 0.08, 0.7)
+    kw = dict(T=T, λρ=1e-3, λμ=1e-2, ν=1.0 * L / 1e4, g=(0.0, 0.0, -1.0 / L), perdir=())
+    return bl, kw
+
+
+def build_sim(ia, N, poisson, rtol):
+    bl, kw = make_case(N, poisson)
+    sim = ia.TwoPhaseSimulation((N, N, N), (0.0, 0.0, 0.0), float(N), poisson=poisson, poisson_rtol=rtol,
+                                poisson_max_outer=200, **kw)
+    s = pool_sdf_samples(N, bl, 0.7)
+    sim._call("ia_apply_vof", ia._ptr(s), s.nbytes)
+    del s
+    sim.flow.Δt[-1] = min(sim.flow.Δt[-1], sim.T.type(sim.MPCFL()))
+    return sim, bl, kw
+
+
+# algorithmic bytes per interior cell (FP32, 3D): each distinct array a kernel
+# must read plus each it must write, once (halo re-reads excluded)
+BYTES_PER_CELL = {
+    "momentum": 4 * 3 + 4 * 3 + 4 + 4 + 4 + 4 + 1 + 4 * 3,  # ρu in/out, f_pre, ρuf_d, u_d, u⁰_d, c̄, uOld
+    "vof": 4 + 4 + 4 + 4 + 1 + 4,                           # f in/out, u_d, u⁰_d, c̄, ρuf_d
+    "visc_update": 4 * 3 + 4 + 4 + 4 * 3 + 4 * 3 + 4 * 3 + 4 * 3,  # u, f_cur, f_upd, u⁰, ρu -> u, L
+    "pcg_Ae": 4 + 4 * 3 + 4 + 4,                            # ϵ, L, D -> z
+    "pcg_update": 4 * 5 + 4 * 3,                            # iD, ϵ, z, r, x -> x, r, z
+    "increment": 4 * 5 + 4 * 2,                             # r, ϵ, L(3), D, x -> r, x  (L counted as 3)
+}
+BYTES_PER_CELL["increment"] = 4 + 4 + 12 + 4 + 4 + 4 + 4
+
+
+def cpu_baseline(N_cpu, poisson, rtol, steps):
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    bl, kw = make_case(N_cpu, poisson)
+    o = O.OracleSim((N_cpu,) * 3, (0.0, 0.0, 0.0), float(N_cpu), mg_tol=1e-4, **kw)
+    s = pool_sdf_samples(N_cpu, bl, 0.7)
+    o.lib.oc_apply_vof(o.ctx, O._ptr(s))
+    o.Δt[-1] = min(o.Δt[-1], o.T.type(o.cfl()))
+    O.lib(o.T).oc_set_threads(threads)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.sim_step()
+    t = time.perf_counter() - t0
+    return dict(value=N_cpu ** 3 * steps / t / 1e6, unit="Mcell-steps/s", cores=threads, kind="port",
+                sample=f"CPU oracle (C, OpenMP, FP32, reference WaterLily solver! tol 1e-4), same pool+bubbles "
+                       f"case at {N_cpu}^3, {steps} step(s) from rest, {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--poisson", default="refined", choices=["refined", "reference", "learned"])
+    ap.add_argument("--rtol", type=float, default=1e-6)
+    ap.add_argument("--profile", default="momentum", help="kernel class whose launches are event-timed")
+    ap.add_argument("--cpu-N", type=int, default=96)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the path shards by z-slabs across GPUs by design, "
+              "but no multi-GPU run is part of it)", file=sys.stderr)
+        sys.exit(2)
+    ia = load_package()
+    N = args.N
+    t0 = time.perf_counter()
+    sim, bl, kw = build_sim(ia, N, args.poisson, args.rtol)
+    print(f"[bench] setup {N}^3 with {len(bl)} bubbles in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        ia.sim_step_(sim)
+    sim._call("ia_synchronize")
+    prof = ia.PROFILE[args.profile]
+    sim._call("ia_profile_enable", prof, 1)
+    iters, solve_ms, rel = [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = ia.MPFMomStep_(sim, sim, sim)
+        for q in range(2):
+            iters.append(st.solve[q].iters)
+            solve_ms.append(st.solve[q].ms)
+            rel.append(st.solve[q].rel)
+    sim._call("ia_synchronize")
+    t = time.perf_counter() - t0
+    nl, ms = ia.C.c_int(), ia.C.c_double()
+    sim._call("ia_profile_read", prof, ia.C.byref(nl), ia.C.byref(ms))
+    sim._call("ia_profile_enable", prof, 0)
+    print(f"[bench] {args.steps} steps in {t:.3f} s; V-cycles per solve {iters}", file=sys.stderr, flush=True)
+    cells = N ** 3
+    value = cells * args.steps / t / 1e6
+    avg_ms = ms.value / max(1, nl.value)
+    bpl = BYTES_PER_CELL[args.profile] * cells
+    achieved = bpl / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3D two-phase pool + {len(bl)} bubbles (seed 3), {N}^3, slip walls, gravity, "
+                               f"density ratio 1000, viscosity ratio 100",
+                   "grid": [N, N, N], "poisson": args.poisson, "poisson_rtol": args.rtol,
+                   "vcycles_per_solve_mean": float(np.mean(iters)),
+                   "poisson_ms_per_solve_mean": round(float(np.mean(solve_ms)), 3),
+                   "poisson_rel_residual_max": float(np.max(rel)) if rel else None},
+        "roofline": {"kernel": args.profile, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "bytes_per_launch": bpl, "avg_launch_ms": round(avg_ms, 4), "launches": nl.value},
+    }
+    sim.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -159,6 +159,21 @@ int ia_mg_update(ia_ctx *c);
 /* myproject!(flow,pois,w) with dt_w = w*Δt, then BC!(u)  flow.jl:328-347, :82 */
 int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats);
 
+/* Kernel-class timing with HIP events on the context's stream (no reference
+ * counterpart; used by the benchmark to measure one kernel's launches inside
+ * a timed region).  Enabling resets the record; MG classes time level 0 only. */
+enum ia_prof_class {
+    IA_PROF_VOF = 0,          /* fused cVOF sweep */
+    IA_PROF_MOMENTUM = 1,     /* fused CMOM/SynDRoM sweep */
+    IA_PROF_VISC_UPDATE = 2,  /* fused viscosity + RK update + coefficients */
+    IA_PROF_PCG_AE = 3,       /* pcg z = Aϵ, finest level */
+    IA_PROF_PCG_UPDATE = 4,   /* pcg x,r,z update, finest level */
+    IA_PROF_INCREMENT = 5,    /* increment! r -= Aϵ, x += ϵ, finest level */
+    IA_PROF_NCLASS = 6
+};
+int ia_profile_enable(ia_ctx *c, int kernel, int on);
+int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms);
+
 int ia_synchronize(ia_ctx *c);
 const char *ia_last_error(ia_ctx *c);
 

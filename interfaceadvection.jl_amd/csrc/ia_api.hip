@@ -5,7 +5,7 @@
 // per pressure V-cycle (the solver's stopping test) and once per step (CFL
 // and fill flags).
 #include "ia.h"
-#include "ia_kernels.h"
+#include "ia_poisson.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 using namespace ia;
 
@@ -22,6 +23,7 @@ namespace {
 struct Level {
     Grid g;
     Tiling t;
+    CTiling ct;
     void *L = nullptr, *Dg = nullptr, *iD = nullptr, *x = nullptr, *e = nullptr, *r = nullptr, *z = nullptr;
     bool own = false;
 };
@@ -48,10 +50,20 @@ struct ia_ctx {
     double *part = nullptr;
     long part_n = 0;
     StepState *st = nullptr;
-    SolveCtl *ctl = nullptr;
+    PcgState *pst = nullptr;   // pcg! state slots
+    FcgState *fst = nullptr;   // flexible-CG scalars
+    double *pp = nullptr;      // partial-sum buffers (PP_N of pp_stride doubles)
+    long pp_stride = 0;
+    std::vector<double> hpart; // host copy of a partial-sum buffer
     double *dsc = nullptr;   // device scalars
-    double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr;
+    double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
+    int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct ProfClass {
+        bool on = false;
+        int n = 0;
+        std::vector<hipEvent_t> ev;
+    } prof[IA_PROF_NCLASS];
     char msg[512] = {0};
 };
 
@@ -77,6 +89,24 @@ int fail(ia_ctx *c, int code, const char *fmt, ...) {
         if (_e != hipSuccess) return fail(c, IA_ERR_HIP, "launch: %s", hipGetErrorString(_e));    \
     } while (0)
 
+// HIP events around the launches of one kernel class (ia_profile_enable)
+void prof_begin(ia_ctx *c, int k) {
+    auto &p = c->prof[k];
+    if (!p.on) return;
+    while ((int)p.ev.size() < 2 * (p.n + 1)) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { p.on = false; return; }
+        p.ev.push_back(e);
+    }
+    (void)hipEventRecord(p.ev[2 * p.n], c->s);
+}
+void prof_end(ia_ctx *c, int k) {
+    auto &p = c->prof[k];
+    if (!p.on) return;
+    (void)hipEventRecord(p.ev[2 * p.n + 1], c->s);
+    p.n++;
+}
+
 void grid_init(Grid &g, int D, const int *Ng, int perdir) {
     g.D = D;
     for (int a = 0; a < 3; ++a) {
@@ -96,6 +126,19 @@ Tiling tiling(const Grid &g) {
     t.nt[2] = n2;
     t.tz = 1;
     t.nblk = (long)t.nt[0] * t.nt[1] * t.nt[2];
+    return t;
+}
+CTiling ctiling(const Grid &g) {
+    CTiling t;
+    int n0 = g.Ng[0] - 2, n1 = g.D > 1 ? g.Ng[1] - 2 : 1, n2 = g.D > 2 ? g.Ng[2] - 2 : 1;
+    t.nt[0] = (n0 + TX - 1) / TX;
+    t.nt[1] = (n1 + TY - 1) / TY;
+    long base = (long)t.nt[0] * t.nt[1];
+    int zb = 1;
+    if (g.D > 2) zb = (int)std::max(1L, std::min(32L, base * n2 / 2048));
+    t.zb = zb;
+    t.nt[2] = g.D > 2 ? (n2 + zb - 1) / zb : 1;
+    t.nblk = base * t.nt[2];
     return t;
 }
 long interior(const Grid &g) {
@@ -141,6 +184,9 @@ int free_fbuf(ia_ctx *c, int a, int b, int d2) {
         if (q != a && q != b && q != d2) return q;
     return -1;
 }
+
+// partial-sum buffers
+enum { PP_A = 0, PP_B, PP_C, PP_MR, PP_ML2, PP_FI, PP_FZ, PP_FR, PP_FRR, PP_FD, PP_FAP, PP_FU, PP_N };
 
 template <class T> T *P(void *v) { return (T *)v; }
 template <class T> const T *P(const void *v) { return (const T *)v; }
@@ -207,6 +253,7 @@ template <class T> struct Impl {
         Level &l0 = c->lev[0];
         l0.g = g;
         l0.t = c->t;
+        l0.ct = ctiling(g);
         l0.L = c->L;
         l0.x = c->p;
         l0.z = c->sig;
@@ -225,6 +272,7 @@ template <class T> struct Impl {
             Level &lv = c->lev[c->nlev];
             grid_init(lv.g, g.D, Na, c->P.perdir);
             lv.t = tiling(lv.g);
+            lv.ct = ctiling(lv.g);
             size_t n = lv.g.ns;
             if (int r = alloc_zero(c, &lv.L, n * g.D * es)) return r;
             void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z};
@@ -239,7 +287,17 @@ template <class T> struct Impl {
         HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
         HIPCHK(c, hipMalloc(&c->fpart, maxblk * sizeof(FillPart)));
         HIPCHK(c, hipMalloc(&c->st, sizeof(StepState)));
-        HIPCHK(c, hipMalloc(&c->ctl, sizeof(SolveCtl)));
+        HIPCHK(c, hipMalloc(&c->pst, 64 * sizeof(PcgState)));
+        HIPCHK(c, hipMemsetAsync(c->pst, 0, 64 * sizeof(PcgState), c->s));
+        HIPCHK(c, hipMalloc(&c->fst, sizeof(FcgState)));
+        HIPCHK(c, hipMemsetAsync(c->fst, 0, sizeof(FcgState), c->s));
+        {
+            long mb = 0;
+            for (int l = 0; l < c->nlev; ++l) mb = std::max(mb, c->lev[l].ct.nblk);
+            c->pp_stride = 2 * mb + 64;
+            HIPCHK(c, hipMalloc(&c->pp, PP_N * c->pp_stride * sizeof(double)));
+            HIPCHK(c, hipMemsetAsync(c->pp, 0, PP_N * c->pp_stride * sizeof(double), c->s));
+        }
         HIPCHK(c, hipMalloc(&c->dsc, 64 * sizeof(double)));
         HIPCHK(c, hipMemsetAsync(c->dsc, 0, 64 * sizeof(double), c->s));
         if (int r = mg_update(c)) return r;
@@ -261,70 +319,93 @@ template <class T> struct Impl {
         LAUNCHCHK(c);
         return IA_OK;
     }
-    static void finsum(ia_ctx *c, long nblk, double *out) {
-        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, c->s, c->part, nblk, out);
+    static LevV<T> view(Level &l) {
+        LevV<T> v;
+        v.g = l.g;
+        v.t = l.ct;
+        v.L = P<T>(l.L);
+        v.Dg = P<T>(l.Dg);
+        v.iD = P<T>(l.iD);
+        v.x = P<T>(l.x);
+        v.e = P<T>(l.e);
+        v.r = P<T>(l.r);
+        v.z = P<T>(l.z);
+        return v;
     }
+    static dim3 cb(const Level &l) { return dim3((unsigned)l.ct.nblk); }
     static void jacobi(ia_ctx *c, Level &l) {
-        hipLaunchKernelGGL(k_jacobi_eps<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.r), P<T>(l.iD), P<T>(l.e));
+        hipLaunchKernelGGL(k_mg_jacobi<T>, cb(l), dim3(NT), 0, c->s, view(l));
         increment(c, l);
     }
     static void increment(ia_ctx *c, Level &l) {
-        hipLaunchKernelGGL(k_increment<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.L), P<T>(l.Dg), P<T>(l.e),
-                           P<T>(l.r), P<T>(l.x));
+        bool f0 = &l == &c->lev[0];
+        if (f0) prof_begin(c, IA_PROF_INCREMENT);
+        hipLaunchKernelGGL(k_mg_increment<T>, cb(l), dim3(NT), 0, c->s, view(l));
+        if (f0) prof_end(c, IA_PROF_INCREMENT);
     }
+    static double *pp(ia_ctx *c, int k) { return c->pp + (long)k * c->pp_stride; }
+    // WaterLily pcg!(p; it)
     static void pcg(ia_ctx *c, Level &l, int it = 6) {
-        long nb = l.t.nblk;
-        hipLaunchKernelGGL(k_pcg_init<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.r), P<T>(l.iD), P<T>(l.z),
-                           P<T>(l.e), c->part);
-        hipLaunchKernelGGL(k_pcg_fin_rho0<T>, dim3(1), dim3(NT), 0, c->s, c->part, nb, c->ctl);
-        for (int q = 1; q <= it; ++q) {
-            hipLaunchKernelGGL(k_pcg_Ae<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.L), P<T>(l.Dg), P<T>(l.e),
-                               P<T>(l.z), c->part, c->ctl);
-            hipLaunchKernelGGL(k_pcg_fin_alpha<T>, dim3(1), dim3(NT), 0, c->s, c->part, nb, c->ctl, it);
-            hipLaunchKernelGGL(k_pcg_update<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.iD), P<T>(l.e),
-                               P<T>(l.z), P<T>(l.r), P<T>(l.x), c->part, c->ctl);
-            hipLaunchKernelGGL(k_pcg_fin_beta<T>, dim3(1), dim3(NT), 0, c->s, c->part, nb, c->ctl);
-            if (q < it)
-                hipLaunchKernelGGL(k_pcg_eps<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.z), P<T>(l.e), c->ctl);
+        LevV<T> v = view(l);
+        bool f0 = &l == &c->lev[0];
+        dim3 B = cb(l);
+        hipLaunchKernelGGL(k_pcg_init<T>, B, dim3(NT), 0, c->s, v, pp(c, PP_A));
+        for (int i = 1; i <= it; ++i) {
+            int sa = 3 * (i - 1), su = sa + 1, se = sa + 2;
+            int sin = i == 1 ? -1 : sa - 1;
+            if (f0) prof_begin(c, IA_PROF_PCG_AE);
+            hipLaunchKernelGGL(k_pcg_Ae<T>, B, dim3(NT), 0, c->s, v, c->pst, sin, sa, pp(c, PP_A), pp(c, PP_B));
+            if (f0) prof_end(c, IA_PROF_PCG_AE);
+            if (f0) prof_begin(c, IA_PROF_PCG_UPDATE);
+            hipLaunchKernelGGL(k_pcg_update<T>, B, dim3(NT), 0, c->s, v, c->pst, sa, su, pp(c, PP_B), pp(c, PP_C), i, it);
+            if (f0) prof_end(c, IA_PROF_PCG_UPDATE);
+            if (i < it) hipLaunchKernelGGL(k_pcg_eps<T>, B, dim3(NT), 0, c->s, v, c->pst, su, se, pp(c, PP_C));
         }
     }
     static void vcycle(ia_ctx *c, int l) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
         jacobi(c, fl);
-        hipLaunchKernelGGL(k_restrict<T>, blocks(co.t), dim3(NT), 0, c->s, co.g, co.t, fl.g, P<T>(fl.r), P<T>(co.r),
-                           P<T>(co.x));
+        hipLaunchKernelGGL(k_mg_restrict<T>, cb(co), dim3(NT), 0, c->s, view(co), fl.g, P<T>(fl.r));
         if (l + 2 < c->nlev) vcycle(c, l + 1);
         pcg(c, co);
-        hipLaunchKernelGGL(k_prolong<T>, blocks(fl.t), dim3(NT), 0, c->s, fl.g, fl.t, co.g, P<T>(co.x), P<T>(fl.e));
+        hipLaunchKernelGGL(k_mg_prolong<T>, cb(fl), dim3(NT), 0, c->s, view(fl), co.g, P<T>(co.x));
         increment(c, fl);
     }
-    static int l2(ia_ctx *c, Level &l, double *out_host) {
-        hipLaunchKernelGGL(k_dot<T>, blocks(l.t), dim3(NT), 0, c->s, l.g, l.t, P<T>(l.r), P<T>(l.r), c->part);
-        finsum(c, l.t.nblk, c->dsc + 0);
-        HIPCHK(c, hipMemcpyAsync(out_host, c->dsc + 0, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    // fixed-order host sum of one partial-sum buffer (K values per block)
+    static int host_total(ia_ctx *c, int buf, long nblk, int K, double *out) {
+        c->hpart.resize(K * nblk);
+        HIPCHK(c, hipMemcpyAsync(c->hpart.data(), pp(c, buf), K * nblk * sizeof(double), hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));
-        *out_host = (double)(T)*out_host;
+        for (int k = 0; k < K; ++k) {
+            double s = 0;
+            for (long q = 0; q < nblk; ++q) s += c->hpart[K * q + k];
+            out[k] = s;
+        }
         return IA_OK;
     }
     // WaterLily solver!(ml; tol, itmx)
     static int solver(ia_ctx *c, double tol, int itmx, int *iters, double *r2o) {
         Level &l0 = c->lev[0];
-        hipLaunchKernelGGL(k_residual<T>, blocks(l0.t), dim3(NT), 0, c->s, l0.g, l0.t, P<T>(l0.L), P<T>(l0.Dg),
-                           P<T>(l0.iD), P<T>(l0.x), P<T>(l0.z), P<T>(l0.r), c->part);
-        hipLaunchKernelGGL(k_fin_mean<T>, dim3(1), dim3(NT), 0, c->s, c->part, l0.t.nblk, interior(l0.g), c->ctl);
-        hipLaunchKernelGGL(k_sub_mean<T>, blocks(l0.t), dim3(NT), 0, c->s, l0.g, l0.t, P<T>(l0.r), c->ctl);
+        long ncell = interior(l0.g), nb = l0.ct.nblk;
+        hipLaunchKernelGGL(k_mg_resid<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR));
+        hipLaunchKernelGGL(k_mg_center_l2<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR), ncell, 1,
+                           pp(c, PP_ML2));
         LAUNCHCHK(c);
         double r2;
-        if (int e = l2(c, l0, &r2)) return e;
+        if (int e = host_total(c, PP_ML2, nb, 1, &r2)) return e;
+        r2 = (double)(T)r2;
         int n = 0;
         while (r2 > tol && n < itmx) {
             vcycle(c, 0);
             pcg(c, l0);
+            hipLaunchKernelGGL(k_mg_center_l2<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR), ncell, 0,
+                               pp(c, PP_ML2));
             LAUNCHCHK(c);
-            if (int e = l2(c, l0, &r2)) return e;
+            if (int e = host_total(c, PP_ML2, nb, 1, &r2)) return e;
+            r2 = (double)(T)r2;
             n++;
         }
-        hipLaunchKernelGGL(k_perbc<T>, blocks(l0.t), dim3(NT), 0, c->s, l0.g, l0.t, P<T>(l0.x));
+        hipLaunchKernelGGL(k_mg_perbc<T>, cb(l0), dim3(NT), 0, c->s, view(l0), P<T>(l0.x));
         LAUNCHCHK(c);
         *iters = n;
         *r2o = r2;
@@ -350,62 +431,86 @@ template <class T> struct Impl {
         }
         return IA_OK;
     }
-    // FP64 iterative refinement around the V-cycle (REFINED / LEARNED)
-    static int resid64(ia_ctx *c, double *rn2_host) {
+    // FP64 flexible CG preconditioned by the FP32 V-cycle (REFINED / LEARNED)
+    static FcgV<T> fview(ia_ctx *c) {
         Level &l0 = c->lev[0];
-        Grid &g = c->g;
-        long nb = c->t.nblk;
-        hipLaunchKernelGGL(k_ir_resid<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(l0.L), P<T>(l0.iD), c->z64,
-                           c->x64, c->r64, c->part);
-        finsum(c, nb, c->dsc + 2);
-        hipLaunchKernelGGL(k_ir_center, blocks(c->t), dim3(NT), 0, c->s, g, c->t, c->r64, c->dsc + 2, interior(g),
-                           c->part);
-        finsum(c, nb, c->dsc + 3);
+        FcgV<T> f;
+        f.g = l0.g;
+        f.t = l0.ct;
+        f.L = P<T>(l0.L);
+        f.iD = P<T>(l0.iD);
+        f.z = c->z64;
+        f.x = c->x64;
+        f.r = c->r64;
+        f.pa = c->pa64;
+        f.pb = c->pb64;
+        f.q = c->q64;
+        f.w = P<T>(l0.x);
+        f.wr = P<T>(l0.r);
+        f.S = c->fst;
+        f.ncell = interior(l0.g);
+        return f;
+    }
+    static int true_resid(ia_ctx *c, double *rr) {
+        Level &l0 = c->lev[0];
+        FcgV<T> f = fview(c);
+        hipLaunchKernelGGL(k_fcg_resid<T>, cb(l0), dim3(NT), 0, c->s, f, pp(c, PP_FR));
+        hipLaunchKernelGGL(k_fcg_center_r<T>, cb(l0), dim3(NT), 0, c->s, f, pp(c, PP_FR), pp(c, PP_FRR));
         LAUNCHCHK(c);
-        HIPCHK(c, hipMemcpyAsync(rn2_host, c->dsc + 3, sizeof(double), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        return IA_OK;
+        return host_total(c, PP_FRR, l0.ct.nblk, 1, rr);
     }
     static int project_refined(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
         Grid &g = c->g;
         Level &l0 = c->lev[0];
-        long nb = c->t.nblk, ncell = interior(g);
+        long nb = l0.ct.nblk;
         if (!c->x64) {
-            HIPCHK(c, hipMalloc(&c->x64, g.ns * sizeof(double)));
-            HIPCHK(c, hipMalloc(&c->r64, g.ns * sizeof(double)));
-            HIPCHK(c, hipMalloc(&c->z64, g.ns * sizeof(double)));
-            HIPCHK(c, hipMemsetAsync(c->x64, 0, g.ns * sizeof(double), c->s));
-            HIPCHK(c, hipMemsetAsync(c->r64, 0, g.ns * sizeof(double), c->s));
-            HIPCHK(c, hipMemsetAsync(c->z64, 0, g.ns * sizeof(double), c->s));
+            double **bufs[] = {&c->x64, &c->r64, &c->z64, &c->pa64, &c->pb64, &c->q64};
+            for (double **b : bufs) {
+                HIPCHK(c, hipMalloc((void **)b, g.ns * sizeof(double)));
+                HIPCHK(c, hipMemsetAsync(*b, 0, g.ns * sizeof(double), c->s));
+            }
         }
-        hipLaunchKernelGGL(k_ir_init<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->p), dtw, c->z64,
-                           c->x64, c->part);
-        finsum(c, nb, c->dsc + 4);
-        hipLaunchKernelGGL(k_ir_center, blocks(c->t), dim3(NT), 0, c->s, g, c->t, c->z64, c->dsc + 4, ncell, c->part);
-        finsum(c, nb, c->dsc + 5);
-        double zn2 = 0, rn2 = 0;
-        HIPCHK(c, hipMemcpyAsync(&zn2, c->dsc + 5, sizeof(double), hipMemcpyDeviceToHost, c->s));
-        if (int e = resid64(c, &rn2)) return e;
-        double rel = zn2 > 0 ? sqrt(rn2 / zn2) : 0.0;
-        int it = 0;
+        FcgV<T> f = fview(c);
+        dim3 B = cb(l0);
+        hipLaunchKernelGGL(k_fcg_init<T>, B, dim3(NT), 0, c->s, f, P<T>(u), P<T>(c->p), dtw, pp(c, PP_FI));
+        hipLaunchKernelGGL(k_fcg_center_z<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FI), pp(c, PP_FZ));
+        double zn2 = 0, rr = 0;
+        if (int e = host_total(c, PP_FZ, nb, 1, &zn2)) return e;
+        if (int e = true_resid(c, &rr)) return e;
+        int rbuf = PP_FRR;  // buffer holding the partials of ‖r‖²
+        double zn = sqrt(zn2);
+        double rel = zn > 0 ? sqrt(rr) / zn : 0.0;
+        int it = 0, k = 0;
         while (rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
-            hipLaunchKernelGGL(k_ir_load<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, c->r64, c->dsc + 3, ncell,
-                               P<T>(l0.r), P<T>(l0.x));
+            f = fview(c);
+            hipLaunchKernelGGL(k_fcg_load<T>, B, dim3(NT), 0, c->s, f, pp(c, rbuf));
             vcycle(c, 0);
-            pcg(c, l0);
-            hipLaunchKernelGGL(k_ir_accum<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(l0.x), c->dsc + 3, ncell,
-                               c->x64);
+            if (c->smooth_it > 0) pcg(c, l0, c->smooth_it);
+            hipLaunchKernelGGL(k_fcg_dots<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), k == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_fcg_Ap<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), pp(c, PP_FAP), k == 0 ? 1 : 0);
+            hipLaunchKernelGGL(k_fcg_update<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FAP), pp(c, PP_FU));
             LAUNCHCHK(c);
-            if (int e = resid64(c, &rn2)) return e;
-            rel = sqrt(rn2 / zn2);
+            std::swap(c->pa64, c->pb64);
+            rbuf = PP_FU;
+            if (int e = host_total(c, PP_FU, nb, 1, &rr)) return e;
+            rel = sqrt(rr) / zn;
             it++;
+            k++;
+            if (rel <= c->P.poisson_rtol) {  // confirm with the true FP64 residual; restart if it drifted
+                if (int e = true_resid(c, &rr)) return e;
+                rbuf = PP_FRR;
+                rel = sqrt(rr) / zn;
+                k = 0;
+            }
         }
-        hipLaunchKernelGGL(k_ir_project<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->L), c->x64,
-                           P<T>(c->p), dtw, (T)c->P.u_bc[0], (T)c->P.u_bc[1], (T)c->P.u_bc[2]);
+        f = fview(c);
+        hipLaunchKernelGGL(k_fcg_project<T>, B, dim3(NT), 0, c->s, f, P<T>(u), P<T>(c->p), dtw, (T)c->P.u_bc[0],
+                           (T)c->P.u_bc[1], (T)c->P.u_bc[2]);
+        hipLaunchKernelGGL(k_mg_perbc<T>, B, dim3(NT), 0, c->s, view(l0), P<T>(c->p));
         LAUNCHCHK(c);
         if (ss) {
             ss->iters = it;
-            ss->r2 = rn2;
+            ss->r2 = rr;
             ss->rel = rel;
         }
         return IA_OK;
@@ -477,7 +582,9 @@ template <class T> struct Impl {
         a.rtol = (T)rtol;
         a.ctol = 10 * (T)epsT();
         a.ubc_d = (T)c->P.u_bc[d];
+        prof_begin(c, IA_PROF_VOF);
         hipLaunchKernelGGL(k_vof_sweep<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        prof_end(c, IA_PROF_VOF);
         hipLaunchKernelGGL(k_fill_finalize<T>, dim3(1), dim3(NT), 0, c->s, g, c->fpart, c->t.nblk, P<T>((void *)u1),
                            P<T>((void *)u2), rtol, c->st);
     }
@@ -499,7 +606,9 @@ template <class T> struct Impl {
         a.dt = dt;
         a.lr = (T)c->P.lambda_rho;
         for (int q = 0; q < 3; ++q) a.ubc[q] = (T)c->P.u_bc[q];
+        prof_begin(c, IA_PROF_MOMENTUM);
         hipLaunchKernelGGL(k_momentum<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        prof_end(c, IA_PROF_MOMENTUM);
         std::swap(c->ru, c->ru2);
     }
     static void dir_order(int D, int step_index, int *dirO) {
@@ -551,7 +660,9 @@ template <class T> struct Impl {
         a.dt = dt;
         a.w = w;
         for (int q = 0; q < 3; ++q) a.g[q] = c->P.has_g ? (T)c->P.g[q] : T(0);
+        prof_begin(c, IA_PROF_VISC_UPDATE);
         hipLaunchKernelGGL(k_visc_update<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        prof_end(c, IA_PROF_VISC_UPDATE);
         LAUNCHCHK(c);
         return IA_OK;
     }
@@ -747,7 +858,8 @@ void ia_destroy(ia_ctx *c) {
     (void)hipSetDevice(c->dev);
     if (c->s) (void)hipStreamSynchronize(c->s);
     void *bufs[] = {c->fb[0], c->fb[1], c->fb[2], c->fb[3], c->u, c->u0, c->ru, c->ru2, c->ruf, c->F, c->p, c->sig,
-                    c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->ctl, c->dsc, c->x64, c->r64, c->z64};
+                    c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->pst, c->fst, c->pp, c->dsc, c->x64, c->r64, c->z64,
+                    c->pa64, c->pb64, c->q64};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int l = 0; l < c->nlev; ++l) {
@@ -761,6 +873,8 @@ void ia_destroy(ia_ctx *c) {
                 if (b) (void)hipFree(b);
         }
     }
+    for (auto &p : c->prof)
+        for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -844,6 +958,27 @@ int ia_synchronize(ia_ctx *c) {
     if (!c) return IA_ERR_BADARG;
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    return IA_OK;
+}
+int ia_profile_enable(ia_ctx *c, int kernel, int on) {
+    if (!c || kernel < 0 || kernel >= IA_PROF_NCLASS) return IA_ERR_BADARG;
+    c->prof[kernel].on = on != 0;
+    c->prof[kernel].n = 0;
+    return IA_OK;
+}
+int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms) {
+    if (!c || kernel < 0 || kernel >= IA_PROF_NCLASS) return IA_ERR_BADARG;
+    auto &p = c->prof[kernel];
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    double t = 0;
+    for (int q = 0; q < p.n; ++q) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, p.ev[2 * q], p.ev[2 * q + 1]));
+        t += ms;
+    }
+    if (launches) *launches = p.n;
+    if (total_ms) *total_ms = t;
     return IA_OK;
 }
 const char *ia_last_error(ia_ctx *c) { return c ? c->msg : "null context"; }

@@ -438,26 +438,28 @@ __global__ void __launch_bounds__(NT) k_update_L(Grid g, Tiling t, const T *f, T
         write_images<T>(g, L + g.ns * i, p, T(1) / linp((f[I] + f[I - g.st[i]]) / 2, lr), 1, i, T(0));
 }
 
-// ---------------------------------------------------------------------------
-// WaterLily MultiLevelPoisson (Poisson.jl / MultiLevelPoisson.jl semantics)
-// ---------------------------------------------------------------------------
+// z = div(u); (flow.jl:345)
+template <class T>
+__global__ void __launch_bounds__(NT) k_div(Grid g, Tiling t, const T *u, T *z) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T s = 0;
+    for (int i = 0; i < g.D; ++i) s += u[g.ns * i + I + g.st[i]] - u[g.ns * i + I];
+    z[I] = s;
+}
+template <class T>
+__global__ void k_scale(T *x, long n, T s) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = x[q] * s;
+}
+template <class T>
+__global__ void k_fill(T *x, long n, T v) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = v;
+}
 template <class T> __device__ inline T xper(const Grid &g, const T *x, int p0, int p1, int p2) {
     return x[(long)map_per(g, 0, p0) + g.st[1] * map_per(g, 1, p1) + g.st[2] * map_per(g, 2, p2)];
-}
-// mult(I,L,D,x) with perBC! applied to x on read
-template <class T> __device__ inline T mg_mult(const Grid &g, const int p[3], long I, const T *L, const T *Dg, const T *x) {
-    T sl = 0, su = 0;
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] -= 1;
-        sl += xper(g, x, q[0], q[1], q[2]) * L[g.ns * i + I];
-    }
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] += 1;
-        su += xper(g, x, q[0], q[1], q[2]) * L[g.ns * i + I + g.st[i]];
-    }
-    return (x[I] * Dg[I] + sl) + su;
 }
 template <class T>
 __global__ void __launch_bounds__(NT) k_set_diag(Grid g, Tiling t, const T *L, T *Dg, T *iD) {
@@ -486,246 +488,11 @@ __global__ void __launch_bounds__(NT) k_restrictL(Grid gc, Tiling t, Grid gf, co
         write_images<T>(gc, Lc + gc.ns * ii, p, (T)(0.5 * (double)s), 1, ii, T(0));
     }
 }
-// z = div(u); (flow.jl:345)
-template <class T>
-__global__ void __launch_bounds__(NT) k_div(Grid g, Tiling t, const T *u, T *z) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    T s = 0;
-    for (int i = 0; i < g.D; ++i) s += u[g.ns * i + I + g.st[i]] - u[g.ns * i + I];
-    z[I] = s;
-}
-template <class T>
-__global__ void k_scale(T *x, long n, T s) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) x[q] = x[q] * s;
-}
-template <class T>
-__global__ void k_fill(T *x, long n, T v) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) x[q] = v;
-}
-// residual! part 1: r = z - A x (0 where iD == 0); block sums of r
-template <class T>
-__global__ void __launch_bounds__(NT) k_residual(Grid g, Tiling t, const T *L, const T *Dg, const T *iD, const T *x,
-                                                 const T *z, T *r, double *part) {
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        T rr = iD[I] == T(0) ? T(0) : z[I] - mg_mult(g, p, I, L, Dg, x);
-        r[I] = rr;
-        v = (double)rr;
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-// residual! part 2: s = Σr / n; flag when |s| > 2eps
-template <class T>
-__global__ void __launch_bounds__(NT) k_fin_mean(const double *part, long nblk, long ncell, SolveCtl *ctl) {
-    double v = 0;
-    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
-    double s = block_sum(v);
-    if (threadIdx.x == 0) {
-        T m = (T)s / (T)ncell;
-        ctl->s = (double)m;
-        ctl->done = fabs(m) <= 2 * eps_of<T>() ? 1 : 0;
-    }
-}
-template <class T>
-__global__ void __launch_bounds__(NT) k_sub_mean(Grid g, Tiling t, T *r, const SolveCtl *ctl) {
-    if (ctl->done) return;
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    r[I] = r[I] - (T)ctl->s;
-}
-// L₂ = r·r (block sums)
-template <class T>
-__global__ void __launch_bounds__(NT) k_dot(Grid g, Tiling t, const T *a, const T *b, double *part) {
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        v = (double)a[I] * (double)b[I];
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
 __global__ void __launch_bounds__(NT) k_fin_sum(const double *part, long nblk, double *out) {
     double v = 0;
     for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
     double s = block_sum(v);
     if (threadIdx.x == 0) *out = s;
-}
-// Jacobi!: ϵ = r·iD
-template <class T>
-__global__ void __launch_bounds__(NT) k_jacobi_eps(Grid g, Tiling t, const T *r, const T *iD, T *e) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    e[I] = r[I] * iD[I];
-}
-// increment!: r -= A ϵ; x += ϵ
-template <class T>
-__global__ void __launch_bounds__(NT) k_increment(Grid g, Tiling t, const T *L, const T *Dg, const T *e, T *r, T *x) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    r[I] = r[I] - mg_mult(g, p, I, L, Dg, e);
-    x[I] = x[I] + e[I];
-}
-// restrict!: coarse r = Σ children
-template <class T>
-__global__ void __launch_bounds__(NT) k_restrict(Grid gc, Tiling t, Grid gf, const T *rf, T *rc, T *xc) {
-    int p[3];
-    if (!thread_cell(gc, t, p)) return;
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = a < gc.D ? 2 * p[a] - 1 : 0;
-        hi[a] = a < gc.D ? 2 * p[a] : 0;
-    }
-    T s = 0;
-    for (int c2 = lo[2]; c2 <= hi[2]; ++c2)
-        for (int c1 = lo[1]; c1 <= hi[1]; ++c1)
-            for (int c0 = lo[0]; c0 <= hi[0]; ++c0) s += rf[c0 + gf.st[1] * c1 + gf.st[2] * c2];
-    long I = lin(gc, p);
-    rc[I] = s;
-    xc[I] = 0;
-}
-// prolongate!: fine ϵ = coarse x[down(I)]
-template <class T>
-__global__ void __launch_bounds__(NT) k_prolong(Grid gf, Tiling t, Grid gc, const T *xc, T *ef) {
-    int p[3];
-    if (!thread_cell(gf, t, p)) return;
-    int q[3];
-    for (int a = 0; a < 3; ++a) q[a] = a < gf.D ? (p[a] + 3) / 2 - 1 : 0;
-    ef[lin(gf, p)] = xc[lin(gc, q)];
-}
-// pcg! (it = 6): init z = ϵ = r·iD, ρ = r·z
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_init(Grid g, Tiling t, const T *r, const T *iD, T *z, T *e, double *part) {
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        T zz = r[I] * iD[I];
-        z[I] = zz;
-        e[I] = zz;
-        v = (double)r[I] * (double)zz;
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_fin_rho0(const double *part, long nblk, SolveCtl *ctl) {
-    double v = 0;
-    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
-    double s = block_sum(v);
-    if (threadIdx.x == 0) {
-        T rho = (T)s;
-        ctl->rho = (double)rho;
-        ctl->done = fabs(rho) < 10 * eps_of<T>() ? 1 : 0;
-        ctl->stop = 0;
-        ctl->it = 0;
-    }
-}
-// z = A ϵ; partial z·ϵ
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_Ae(Grid g, Tiling t, const T *L, const T *Dg, const T *e, T *z, double *part,
-                                               const SolveCtl *ctl) {
-    if (ctl->done) return;
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        T zz = mg_mult(g, p, I, L, Dg, e);
-        z[I] = zz;
-        v = (double)zz * (double)e[I];
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_fin_alpha(const double *part, long nblk, SolveCtl *ctl, int it) {
-    if (ctl->done) return;
-    double v = 0;
-    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
-    double s = block_sum(v);
-    if (threadIdx.x == 0) {
-        T alpha = (T)ctl->rho / (T)s;
-        ctl->alpha = (double)alpha;
-        ctl->it += 1;
-        ctl->stop = (ctl->it == it || fabs((double)alpha) < 1e-2) ? 1 : 0;
-    }
-}
-// x += αϵ; r -= αz; then z = r·iD and partial r·z
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_update(Grid g, Tiling t, const T *iD, const T *e, T *z, T *r, T *x,
-                                                   double *part, const SolveCtl *ctl) {
-    if (ctl->done) return;
-    T alpha = (T)ctl->alpha;
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        x[I] += alpha * e[I];
-        T rr = r[I] - alpha * z[I];
-        r[I] = rr;
-        T zz = rr * iD[I];
-        z[I] = zz;
-        v = (double)rr * (double)zz;
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_fin_beta(const double *part, long nblk, SolveCtl *ctl) {
-    if (ctl->done) return;
-    double v = 0;
-    for (long b = threadIdx.x; b < nblk; b += NT) v += part[b];
-    double s = block_sum(v);
-    if (threadIdx.x == 0) {
-        if (ctl->stop) { ctl->done = 1; return; }
-        T rho2 = (T)s;
-        if (fabs(rho2) < 10 * eps_of<T>()) { ctl->done = 1; return; }
-        T beta = rho2 / (T)ctl->rho;
-        ctl->beta = (double)beta;
-        ctl->rho = (double)rho2;
-    }
-}
-template <class T>
-__global__ void __launch_bounds__(NT) k_pcg_eps(Grid g, Tiling t, const T *z, T *e, const SolveCtl *ctl) {
-    if (ctl->done) return;
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    e[I] = (T)ctl->beta * e[I] + z[I];
-}
-// perBC!(x)
-template <class T>
-__global__ void __launch_bounds__(NT) k_perbc(Grid g, Tiling t, T *x) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    bool any = false;
-    for (int a = 0; a < g.D; ++a) any |= g.per[a] && (p[a] == 1 || p[a] == g.Ng[a] - 2);
-    if (!any) return;
-    // periodic images only (non-periodic ghosts keep their values)
-    int alt[3][2], na[3];
-    for (int a = 0; a < 3; ++a) {
-        na[a] = 1;
-        alt[a][0] = p[a];
-        if (a < g.D && g.per[a]) {
-            if (p[a] == 1) { alt[a][1] = g.Ng[a] - 1; na[a] = 2; }
-            else if (p[a] == g.Ng[a] - 2) { alt[a][1] = 0; na[a] = 2; }
-        }
-    }
-    T v = x[lin(g, p)];
-    for (int i2 = 0; i2 < na[2]; ++i2)
-        for (int i1 = 0; i1 < na[1]; ++i1)
-            for (int i0 = 0; i0 < na[0]; ++i0)
-                x[(long)alt[0][i0] + g.st[1] * alt[1][i1] + g.st[2] * alt[2][i2]] = v;
 }
 // myproject!: u -= L ∂x ; BC!(u)  (flow.jl:331-333, :82)
 template <class T>
@@ -834,112 +601,6 @@ __global__ void __launch_bounds__(NT) k_reconstruct(Grid g, Tiling t, const T *f
                 al[J] = a;
                 for (int d = 0; d < g.D; ++d) nh[g.ns * d + J] = n[d];
             }
-}
-
-// ---------------------------------------------------------------------------
-// FP64 iterative refinement around the FP32 V-cycle (REFINED mode)
-// ---------------------------------------------------------------------------
-template <class T> __device__ inline double mult64(const Grid &g, const int p[3], long I, const T *L, const double *x) {
-    double dg = 0, sl = 0, su = 0;
-    for (int i = 0; i < g.D; ++i) dg -= ((double)L[g.ns * i + I] + (double)L[g.ns * i + I + g.st[i]]);
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] -= 1;
-        sl += x[(long)map_per(g, 0, q[0]) + g.st[1] * map_per(g, 1, q[1]) + g.st[2] * map_per(g, 2, q[2])] *
-              (double)L[g.ns * i + I];
-    }
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] += 1;
-        su += x[(long)map_per(g, 0, q[0]) + g.st[1] * map_per(g, 1, q[1]) + g.st[2] * map_per(g, 2, q[2])] *
-              (double)L[g.ns * i + I + g.st[i]];
-    }
-    return (x[I] * dg + sl) + su;
-}
-// z64 = div u (double); x64 = p·dt; block sums of z
-template <class T>
-__global__ void __launch_bounds__(NT) k_ir_init(Grid g, Tiling t, const T *u, const T *pT, double dtw, double *z,
-                                                double *x, double *part) {
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        double s = 0;
-        for (int i = 0; i < g.D; ++i) s += (double)u[g.ns * i + I + g.st[i]] - (double)u[g.ns * i + I];
-        z[I] = s;
-        x[I] = (double)pT[I] * dtw;
-        v = s;
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-// a -= mean (mean = *sum / n); block sums of a²
-__global__ void __launch_bounds__(NT) k_ir_center(Grid g, Tiling t, double *a, const double *sum, long ncell,
-                                                  double *part) {
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        double m = *sum / (double)ncell;
-        double q = a[I] - m;
-        a[I] = q;
-        v = q * q;
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-// r64 = z - A x (0 where iD == 0); block sums of r
-template <class T>
-__global__ void __launch_bounds__(NT) k_ir_resid(Grid g, Tiling t, const T *L, const T *iD, const double *z,
-                                                 const double *x, double *r, double *part) {
-    int p[3];
-    double v = 0;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        double rr = iD[I] == T(0) ? 0.0 : z[I] - mult64(g, p, I, L, x);
-        r[I] = rr;
-        v = rr;
-    }
-    double s = block_sum(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-// T-precision residual scaled to unit RMS; x_T = 0
-template <class T>
-__global__ void __launch_bounds__(NT) k_ir_load(Grid g, Tiling t, const double *r, const double *nrm2, long ncell,
-                                                T *rT, T *xT) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    double sc = sqrt(*nrm2) / sqrt((double)ncell);
-    rT[I] = (T)(r[I] / sc);
-    xT[I] = T(0);
-}
-// x64 += sc·x_T  (periodic ghosts of x_T are refreshed by the caller)
-template <class T>
-__global__ void __launch_bounds__(NT) k_ir_accum(Grid g, Tiling t, const T *xT, const double *nrm2, long ncell,
-                                                 double *x) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    double sc = sqrt(*nrm2) / sqrt((double)ncell);
-    x[I] += sc * (double)xT[I];
-}
-// u -= L ∂x64 ; p = x64/dt ; BC!(u)
-template <class T>
-__global__ void __launch_bounds__(NT) k_ir_project(Grid g, Tiling t, T *u, const T *L, const double *x, T *pT,
-                                                   double dtw, T ubc0, T ubc1, T ubc2) {
-    int p[3];
-    if (!thread_cell(g, t, p)) return;
-    long I = lin(g, p);
-    T ub[3] = {ubc0, ubc1, ubc2};
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] -= 1;
-        double xm = x[(long)map_per(g, 0, q[0]) + g.st[1] * map_per(g, 1, q[1]) + g.st[2] * map_per(g, 2, q[2])];
-        T un = (T)((double)u[g.ns * i + I] - (double)L[g.ns * i + I] * (x[I] - xm));
-        write_images<T>(g, u + g.ns * i, p, un, 1, i, ub[i]);
-    }
-    pT[I] = (T)(x[I] / dtw);
 }
 
 }  // namespace ia

@@ -580,6 +580,7 @@ struct oc_ctx {
     int nlev;
     double mg_tol;
     int mg_itmx;
+    int coarse_it; /* pcg! iterations on coarse levels inside Vcycle! (WaterLily: 6) */
     char msg[512];
 };
 
@@ -722,7 +723,7 @@ static void mg_vcycle(oc_ctx *c, int l) {
     mg_restrict(co, fl);
     for (size_t q = 0; q < co->g.ns; ++q) co->x[q] = 0;
     if (l + 2 < c->nlev) mg_vcycle(c, l + 1);
-    mg_pcg(co, 6);
+    mg_pcg(co, c->coarse_it);
     mg_prolongate(fl, co);
     mg_increment(fl);
 }
@@ -1299,6 +1300,7 @@ oc_ctx *oc_create(const oc_params *p) {
     c->limiter = p->limiter;
     c->mg_tol = p->mg_tol > 0 ? p->mg_tol : 1e-4;
     c->mg_itmx = p->mg_itmx > 0 ? p->mg_itmx : 200;
+    c->coarse_it = 6;
     size_t n = g->ns, nv = n * g->D;
     /* WaterLily Flow: u = uλ(i,x) = u_BC[i]; BC!; u⁰ = copy(u); μ₀ = 1; BC!(μ₀,0) */
     c->u = alloc_fill(nv, 0);
@@ -1558,6 +1560,128 @@ int oc_project_refined(oc_ctx *c, double dt_w, double rtol, int max_outer, int *
     if (outer) *outer = it;
     if (rel) *rel = rl;
     free(z); free(x); free(r);
+    return OC_OK;
+}
+
+
+
+/* symmetric V(npre,npost) cycle with damped Jacobi smoothing on WaterLily's
+ * levels (same transfer and coarse operators); the coarsest level gets
+ * `ncoarse` smoothing sweeps.  Preconditioner of the flexible CG. */
+static void mg_djacobi(lev_t *lv, T omega, int sweeps) {
+    grid_t *g = &lv->g;
+    rng_t R = r_in(g);
+    for (int s = 0; s < sweeps; ++s) {
+        PFOR_R(R) { size_t I = IX(i, j, k); lv->eps[I] = omega * lv->r[I] * lv->iD[I]; }
+        mg_increment(lv);
+    }
+}
+static void mg_vcycle_sym(oc_ctx *c, int l, int npre, int npost, T omega, int ncoarse) {
+    lev_t *fl = &c->lev[l];
+    if (l == c->nlev - 1) { mg_djacobi(fl, omega, ncoarse); return; }
+    lev_t *co = &c->lev[l + 1];
+    mg_djacobi(fl, omega, npre);
+    mg_restrict(co, fl);
+    for (size_t q = 0; q < co->g.ns; ++q) co->x[q] = 0;
+    mg_vcycle_sym(c, l + 1, npre, npost, omega, ncoarse);
+    mg_prolongate(fl, co);
+    mg_increment(fl);
+    mg_djacobi(fl, omega, npost);
+}
+int g_cycle_type = 0, g_npre = 2, g_npost = 2, g_ncoarse = 16;
+double g_omega = 0.8;
+void oc_set_coarse_it(oc_ctx *c, int it) { c->coarse_it = it; }
+void oc_set_cycle(int type, int npre, int npost, double omega, int ncoarse) {
+    g_cycle_type = type; g_npre = npre; g_npost = npost; g_omega = omega; g_ncoarse = ncoarse;
+}
+/* Flexible preconditioned CG in FP64 (Notay's FCG(1)) with the FP32
+ * MultiLevelPoisson V-cycle (+ `smooth_it` pcg! iterations on the finest
+ * level) as the preconditioner.  This build's solver for the 1e-6 relative
+ * residual target; no reference counterpart.  Returns the iteration count. */
+static void precond(oc_ctx *c, const double *r, double *w, double sc, int smooth_it) {
+    lev_t *l0 = &c->lev[0];
+    size_t n = c->g.ns;
+    for (size_t q = 0; q < n; ++q) { l0->r[q] = (T)(r[q] / sc); l0->x[q] = 0; }
+    if (g_cycle_type == 1) {
+        mg_vcycle_sym(c, 0, g_npre, g_npost, (T)g_omega, g_ncoarse);
+    } else {
+        mg_vcycle(c, 0);
+        if (smooth_it > 0) mg_pcg(l0, smooth_it);
+    }
+    per_bc(&c->g, l0->x);
+    for (size_t q = 0; q < n; ++q) w[q] = sc * (double)l0->x[q];
+}
+static double dot64(const grid_t *g, const double *a, const double *b) {
+    rng_t R = r_in(g);
+    double s = 0;
+    FOR_R(R) { size_t I = IX(i, j, k); s += a[I] * b[I]; }
+    return s;
+}
+int oc_project_fcg(oc_ctx *c, double dt_w, double rtol, int max_it, int smooth_it, int *iters, double *rel) {
+    grid_t *g = &c->g;
+    rng_t R = r_in(g);
+    lev_t *l0 = &c->lev[0];
+    size_t n = g->ns;
+    long cnt = rng_count(R);
+    double *z = calloc(n, 8), *x = calloc(n, 8), *r = calloc(n, 8), *w = calloc(n, 8), *p = calloc(n, 8),
+           *q = calloc(n, 8);
+    double zs = 0;
+    FOR_R(R) {
+        size_t I = IX(i, j, k);
+        double s = 0;
+        for (int d = 1; d <= g->D; ++d) s += (double)VC(c->u, d)[I + ST(d)] - (double)VC(c->u, d)[I];
+        z[I] = s;
+        zs += s;
+    }
+    zs /= (double)cnt;
+    double zn = 0;
+    FOR_R(R) { size_t I = IX(i, j, k); z[I] -= zs; zn += z[I] * z[I]; }
+    zn = sqrt(zn);
+    for (size_t qq = 0; qq < n; ++qq) x[qq] = (double)c->p[qq] * dt_w;
+    per_bc64(g, x);
+    double rn = resid64(c, z, x, r);
+    double rl = zn > 0 ? rn / zn : 0.0;
+    int it = 0;
+    double pq = 0;
+    while (rl > rtol && it < max_it) {
+        precond(c, r, w, rn / sqrt((double)cnt), smooth_it);
+        double wr = dot64(g, w, r);
+        double beta = 0;
+        if (it > 0) beta = -dot64(g, w, q) / pq;
+        FOR_R(R) { size_t I = IX(i, j, k); p[I] = w[I] + beta * p[I]; }
+        per_bc64(g, p);
+        PFOR_R(R) { size_t I = IX(i, j, k); PCOORD(P); (void)P; q[I] = mult64(g, I, l0->L, p); }
+        pq = dot64(g, p, q);
+        double alpha = wr / pq;
+        double nn = 0;
+        FOR_R(R) {
+            size_t I = IX(i, j, k);
+            x[I] += alpha * p[I];
+            r[I] -= alpha * q[I];
+            nn += r[I] * r[I];
+        }
+        rn = sqrt(nn);
+        rl = rn / zn;
+        it++;
+        if (rl <= rtol) {  /* confirm with the true residual */
+            per_bc64(g, x);
+            rn = resid64(c, z, x, r);
+            rl = rn / zn;
+        }
+    }
+    per_bc64(g, x);
+    for (int i_ = 1; i_ <= g->D; ++i_) {
+        T *ui = VC(c->u, i_), *Li = VC(c->mu0, i_);
+        PFOR_R(R) {
+            size_t I = IX(i, j, k);
+            ui[I] = (T)((double)ui[I] - (double)Li[I] * (x[I] - x[I - ST(i_)]));
+        }
+    }
+    for (size_t qq = 0; qq < n; ++qq) c->p[qq] = (T)(x[qq] / dt_w);
+    bc_vec(g, c->u, c->uBC);
+    if (iters) *iters = it;
+    if (rel) *rel = rl;
+    free(z); free(x); free(r); free(w); free(p); free(q);
     return OC_OK;
 }
 

@@ -84,6 +84,8 @@ int oc_mg_solve(oc_ctx *c, double tol, int itmx, int *iters, double *r2);
 /* iterative refinement variant: FP64 outer residual/accumulation around the
  * MultiLevelPoisson V-cycles; stops at ||z-Ax||2/||z||2 <= rtol */
 int oc_project_refined(oc_ctx *c, double dt_w, double rtol, int max_outer, int *outer, double *rel);
+/* FP64 flexible CG (FCG(1)) preconditioned by the FP32 V-cycle + smooth_it fine pcg! iterations */
+int oc_project_fcg(oc_ctx *c, double dt_w, double rtol, int max_it, int smooth_it, int *iters, double *rel);
 
 /* point functions (PLIC.jl, normalEstimation.jl, VOFutil.jl) for the KATs */
 real_t oc_getIntercept2(real_t n1, real_t n2, real_t g);

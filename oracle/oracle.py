@@ -70,6 +70,7 @@ def lib(T):
         "oc_project": (C.c_int, [vp, C.c_double, ip, dp]),
         "oc_mg_solve": (C.c_int, [vp, C.c_double, C.c_int, ip, dp]),
         "oc_project_refined": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, ip, dp]),
+        "oc_project_fcg": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, ip, dp]),
         "oc_getIntercept2": (R, [R, R, R]),
         "oc_getIntercept3": (R, [R, R, R, R]),
         "oc_getVolumeFraction2": (R, [R, R, R]),
@@ -227,6 +228,12 @@ class OracleSim:
     def project_refined(self, dt_w, rtol=1e-6, max_outer=100):
         it, rel = C.c_int(), C.c_double()
         self.lib.oc_project_refined(self.ctx, float(dt_w), float(rtol), int(max_outer), C.byref(it), C.byref(rel))
+        return it.value, rel.value
+
+    def project_fcg(self, dt_w, rtol=1e-6, max_it=200, smooth_it=6):
+        it, rel = C.c_int(), C.c_double()
+        self.lib.oc_project_fcg(self.ctx, float(dt_w), float(rtol), int(max_it), int(smooth_it), C.byref(it),
+                                C.byref(rel))
         return it.value, rel.value
 
     def cfl(self):

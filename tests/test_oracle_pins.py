@@ -207,6 +207,29 @@ def test_poisson_refined_reaches_tolerance():
     assert np.linalg.norm(dv - dv.mean()) < 1e-4 * np.linalg.norm(u)
 
 
+def test_poisson_fcg_matches_direct_solve():
+    """FP64 flexible CG with the FP32 V-cycle preconditioner converges to the
+    sparse direct solution of the same operator."""
+    rng = np.random.default_rng(9)
+    N = 32
+    sdf = lambda x: np.sqrt((x[0] - 15.3) ** 2 + (x[1] - 12.1) ** 2) - 6.2
+    sim = O.OracleSim((N, N), (0.0, 0.0), T=np.float32, InterfaceSDF=sdf, λρ=1e-3)
+    u = np.zeros(sim.Ng + (2,), dtype=np.float32, order="F")
+    u[1:-1, 1:-1, :] = rng.standard_normal((N, N, 2))
+    sim.lib.oc_set_initial_velocity(sim.ctx, O._ptr(u))
+    sim.lib.oc_update_L(sim.ctx, 0)
+    sim.lib.oc_mg_update(sim.ctx)
+    z = _div(sim.get(O.U).astype(np.float64), N)
+    it, rel = sim.project_fcg(1.0, 1e-8, 200, 1)
+    assert rel <= 1e-8
+    A = _poisson_matrix(sim)
+    zz = z.ravel(order="F") - z.mean()
+    sol = spla.lsqr(A, zz, atol=1e-15, btol=1e-15, iter_lim=50000)[0]
+    x = sim.get(O.P)[1:-1, 1:-1].astype(np.float64).ravel(order="F")
+    sol = sol - sol.mean() + x.mean()
+    assert np.linalg.norm(x - sol) / np.linalg.norm(sol) < 1e-5
+
+
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_uniform_flow_stays_uniform(T):
     """CMOM consistency: momentum flux = U x mass flux, so a uniform velocity
