@@ -11,6 +11,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -55,9 +56,13 @@ struct ia_ctx {
     double *pp = nullptr;      // partial-sum buffers (PP_N of pp_stride doubles)
     long pp_stride = 0;
     std::vector<double> hpart; // host copy of a partial-sum buffer
+    hipGraphExec_t fcg_graph[4] = {};   // one flexible-CG iteration: [first][p-buffer parity]
+    bool capturing = false;
+    int use_graphs = 0;    // IA_USE_GRAPHS=1: replay each FCG iteration as a HIP graph
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
     int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
+    int tail_level = 99; // first level handled by the one-workgroup tail kernel
     hipEvent_t e0 = nullptr, e1 = nullptr;
     struct ProfClass {
         bool on = false;
@@ -92,7 +97,7 @@ int fail(ia_ctx *c, int code, const char *fmt, ...) {
 // HIP events around the launches of one kernel class (ia_profile_enable)
 void prof_begin(ia_ctx *c, int k) {
     auto &p = c->prof[k];
-    if (!p.on) return;
+    if (!p.on || c->capturing) return;
     while ((int)p.ev.size() < 2 * (p.n + 1)) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) { p.on = false; return; }
@@ -102,7 +107,7 @@ void prof_begin(ia_ctx *c, int k) {
 }
 void prof_end(ia_ctx *c, int k) {
     auto &p = c->prof[k];
-    if (!p.on) return;
+    if (!p.on || c->capturing) return;
     (void)hipEventRecord(p.ev[2 * p.n + 1], c->s);
     p.n++;
 }
@@ -283,6 +288,10 @@ template <class T> struct Impl {
             c->nlev++;
         }
         if (c->nlev <= 2) return fail(c, IA_ERR_BADARG, "MultiLevelPoisson requires size=a2^n, where n>2");
+        c->tail_level = c->nlev;
+        for (int l = 1; l < c->nlev; ++l)
+            if (interior(c->lev[l].g) <= 4096) { c->tail_level = l; break; }
+        if (const char *tv = getenv("IA_TAIL_LEVEL")) c->tail_level = std::max(1, atoi(tv));
         c->part_n = 2 * maxblk + 64;
         HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
         HIPCHK(c, hipMalloc(&c->fpart, maxblk * sizeof(FillPart)));
@@ -362,12 +371,32 @@ template <class T> struct Impl {
             if (i < it) hipLaunchKernelGGL(k_pcg_eps<T>, B, dim3(NT), 0, c->s, v, c->pst, su, se, pp(c, PP_C));
         }
     }
+    // Vcycle!(ml; l): the levels from c->tail_level down run in one workgroup
     static void vcycle(ia_ctx *c, int l) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
         jacobi(c, fl);
         hipLaunchKernelGGL(k_mg_restrict<T>, cb(co), dim3(NT), 0, c->s, view(co), fl.g, P<T>(fl.r));
-        if (l + 2 < c->nlev) vcycle(c, l + 1);
-        pcg(c, co);
+        if (l + 1 >= c->tail_level) {
+            TailArgs<T> a;
+            for (int q = 0; q < c->nlev && q < MAXLEV; ++q) {
+                Level &lv = c->lev[q];
+                a.lv[q].g = lv.g;
+                a.lv[q].L = P<T>(lv.L);
+                a.lv[q].Dg = P<T>(lv.Dg);
+                a.lv[q].iD = P<T>(lv.iD);
+                a.lv[q].x = P<T>(lv.x);
+                a.lv[q].e = P<T>(lv.e);
+                a.lv[q].r = P<T>(lv.r);
+                a.lv[q].z = P<T>(lv.z);
+                a.lv[q].n = (int)interior(lv.g);
+            }
+            a.l0 = l + 1;
+            a.nlev = c->nlev;
+            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), 0, c->s, a);
+        } else {
+            if (l + 2 < c->nlev) vcycle(c, l + 1);
+            pcg(c, co);
+        }
         hipLaunchKernelGGL(k_mg_prolong<T>, cb(fl), dim3(NT), 0, c->s, view(fl), co.g, P<T>(co.x));
         increment(c, fl);
     }
@@ -432,7 +461,7 @@ template <class T> struct Impl {
         return IA_OK;
     }
     // FP64 flexible CG preconditioned by the FP32 V-cycle (REFINED / LEARNED)
-    static FcgV<T> fview(ia_ctx *c) {
+    static FcgV<T> fview(ia_ctx *c, int par = 0) {
         Level &l0 = c->lev[0];
         FcgV<T> f;
         f.g = l0.g;
@@ -442,8 +471,8 @@ template <class T> struct Impl {
         f.z = c->z64;
         f.x = c->x64;
         f.r = c->r64;
-        f.pa = c->pa64;
-        f.pb = c->pb64;
+        f.pa = par ? c->pb64 : c->pa64;
+        f.pb = par ? c->pa64 : c->pb64;
         f.q = c->q64;
         f.w = P<T>(l0.x);
         f.wr = P<T>(l0.r);
@@ -458,6 +487,41 @@ template <class T> struct Impl {
         hipLaunchKernelGGL(k_fcg_center_r<T>, cb(l0), dim3(NT), 0, c->s, f, pp(c, PP_FR), pp(c, PP_FRR));
         LAUNCHCHK(c);
         return host_total(c, PP_FRR, l0.ct.nblk, 1, rr);
+    }
+    // one flexible-CG iteration: preconditioner (V-cycle + pcg! smoothing) and
+    // the FP64 Krylov update.  The input residual's partial sums are in
+    // buffer `rbuf` (the true residual after a restart, else the last update).
+    static void fcg_body(ia_ctx *c, int first, int par, int rbuf) {
+        Level &l0 = c->lev[0];
+        FcgV<T> f = fview(c, par);
+        dim3 B = cb(l0);
+        hipLaunchKernelGGL(k_fcg_load<T>, B, dim3(NT), 0, c->s, f, pp(c, rbuf));
+        vcycle(c, 0);
+        if (c->smooth_it > 0) pcg(c, l0, c->smooth_it);
+        hipLaunchKernelGGL(k_fcg_dots<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), first);
+        hipLaunchKernelGGL(k_fcg_Ap<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), pp(c, PP_FAP), first);
+        hipLaunchKernelGGL(k_fcg_update<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FAP), pp(c, PP_FU));
+    }
+    static int fcg_iteration(ia_ctx *c, int first, int par, int rbuf) {
+        if (!c->use_graphs || rbuf != (first ? PP_FRR : PP_FU)) {
+            fcg_body(c, first, par, rbuf);
+            LAUNCHCHK(c);
+            return IA_OK;
+        }
+        hipGraphExec_t &ex = c->fcg_graph[2 * first + par];
+        if (!ex) {
+            hipGraph_t gr;
+            c->capturing = true;
+            HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+            fcg_body(c, first, par, rbuf);
+            hipError_t e1 = hipStreamEndCapture(c->s, &gr);
+            c->capturing = false;
+            HIPCHK(c, e1);
+            HIPCHK(c, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+            HIPCHK(c, hipGraphDestroy(gr));
+        }
+        HIPCHK(c, hipGraphLaunch(ex, c->s));
+        return IA_OK;
     }
     static int project_refined(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
         Grid &g = c->g;
@@ -481,16 +545,10 @@ template <class T> struct Impl {
         double zn = sqrt(zn2);
         double rel = zn > 0 ? sqrt(rr) / zn : 0.0;
         int it = 0, k = 0;
+        int par = 0;
         while (rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
-            f = fview(c);
-            hipLaunchKernelGGL(k_fcg_load<T>, B, dim3(NT), 0, c->s, f, pp(c, rbuf));
-            vcycle(c, 0);
-            if (c->smooth_it > 0) pcg(c, l0, c->smooth_it);
-            hipLaunchKernelGGL(k_fcg_dots<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), k == 0 ? 1 : 0);
-            hipLaunchKernelGGL(k_fcg_Ap<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), pp(c, PP_FAP), k == 0 ? 1 : 0);
-            hipLaunchKernelGGL(k_fcg_update<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FAP), pp(c, PP_FU));
-            LAUNCHCHK(c);
-            std::swap(c->pa64, c->pb64);
+            if (int e = fcg_iteration(c, k == 0 ? 1 : 0, par, rbuf)) return e;
+            par ^= 1;
             rbuf = PP_FU;
             if (int e = host_total(c, PP_FU, nb, 1, &rr)) return e;
             rel = sqrt(rr) / zn;
@@ -833,6 +891,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     if (c->P.poisson_max_outer <= 0) c->P.poisson_max_outer = 100;
     c->dev = p->device;
     c->es = p->dtype == IA_F64 ? 8 : 4;
+    if (const char *gv = getenv("IA_USE_GRAPHS")) c->use_graphs = atoi(gv) != 0;
     int Ng[3];
     for (int a = 0; a < 3; ++a) Ng[a] = a < p->D ? p->N[a] + 2 : 1;
     grid_init(c->g, p->D, Ng, p->perdir);
@@ -875,6 +934,8 @@ void ia_destroy(ia_ctx *c) {
     }
     for (auto &p : c->prof)
         for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
+    for (auto &ex : c->fcg_graph)
+        if (ex) (void)hipGraphExecDestroy(ex);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->s) (void)hipStreamDestroy(c->s);

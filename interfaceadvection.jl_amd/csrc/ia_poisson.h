@@ -78,20 +78,49 @@ template <class T> struct LevV {
     T *x, *e, *r, *z;
 };
 
-// mult(I,L,D,x) with perBC! on read
-template <class T> __device__ inline T mgmult(const Grid &g, const int p[3], long I, const T *L, const T *Dg, const T *x) {
+// neighbour offsets of an interior cell: ±δ on every axis, wrapped on a
+// periodic axis at the boundary (perBC! on read); ghosts of non-periodic
+// axes are read as stored (their coefficient L is zero)
+struct Nb {
+    long m[3], p[3];
+};
+template <int D> __device__ inline Nb neighbours_d(const Grid &g, const int p[3]) {
+    Nb n;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        long s = g.st[a];
+        n.m[a] = -s;
+        n.p[a] = s;
+        if (g.per[a]) {
+            long w = (long)(g.Ng[a] - 3) * s;
+            if (p[a] == 1) n.m[a] = w;
+            if (p[a] == g.Ng[a] - 2) n.p[a] = -w;
+        }
+    }
+    return n;
+}
+// mult(I,L,D,x)  (WaterLily: x[I]D[I] + Σ_i x[I-δi]L[I,i] + Σ_i x[I+δi]L[I+δi,i])
+template <int D, class T>
+__device__ inline T mgmult_d(const Grid &g, const int p[3], long I, const T *__restrict__ L, const T *__restrict__ Dg,
+                             const T *__restrict__ x) {
+    Nb n = neighbours_d<D>(g, p);
+    T xm[D], xp[D], lm[D], lp[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        xm[i] = x[I + n.m[i]];
+        xp[i] = x[I + n.p[i]];
+        lm[i] = L[g.ns * i + I];
+        lp[i] = L[g.ns * i + I + g.st[i]];
+    }
     T sl = 0, su = 0;
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] -= 1;
-        sl += xper(g, x, q[0], q[1], q[2]) * L[g.ns * i + I];
-    }
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] += 1;
-        su += xper(g, x, q[0], q[1], q[2]) * L[g.ns * i + I + g.st[i]];
-    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) sl += xm[i] * lm[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i) su += xp[i] * lp[i];
     return (x[I] * Dg[I] + sl) + su;
+}
+template <class T> __device__ inline T mgmult(const Grid &g, const int p[3], long I, const T *L, const T *Dg, const T *x) {
+    return g.D == 3 ? mgmult_d<3>(g, p, I, L, Dg, x) : mgmult_d<2>(g, p, I, L, Dg, x);
 }
 
 #define COL_LOOP(lv, ...)                                           \
@@ -297,20 +326,29 @@ template <class T> struct FcgV {
     FcgState *S;
     long ncell;
 };
-template <class T> __device__ inline double mult64(const Grid &g, const int p[3], long I, const T *L, const double *x) {
+template <int D, class T>
+__device__ inline double mult64_d(const Grid &g, const int p[3], long I, const T *__restrict__ L,
+                                  const double *__restrict__ x) {
+    Nb n = neighbours_d<D>(g, p);
+    double lm[D], lp[D], xm[D], xp[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        lm[i] = (double)L[g.ns * i + I];
+        lp[i] = (double)L[g.ns * i + I + g.st[i]];
+        xm[i] = x[I + n.m[i]];
+        xp[i] = x[I + n.p[i]];
+    }
     double dg = 0, sl = 0, su = 0;
-    for (int i = 0; i < g.D; ++i) dg -= ((double)L[g.ns * i + I] + (double)L[g.ns * i + I + g.st[i]]);
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] -= 1;
-        sl += xper(g, x, q[0], q[1], q[2]) * (double)L[g.ns * i + I];
-    }
-    for (int i = 0; i < g.D; ++i) {
-        int q[3] = {p[0], p[1], p[2]};
-        q[i] += 1;
-        su += xper(g, x, q[0], q[1], q[2]) * (double)L[g.ns * i + I + g.st[i]];
-    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) dg -= (lm[i] + lp[i]);
+#pragma unroll
+    for (int i = 0; i < D; ++i) sl += xm[i] * lm[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i) su += xp[i] * lp[i];
     return (x[I] * dg + sl) + su;
+}
+template <class T> __device__ inline double mult64(const Grid &g, const int p[3], long I, const T *L, const double *x) {
+    return g.D == 3 ? mult64_d<3>(g, p, I, L, x) : mult64_d<2>(g, p, I, L, x);
 }
 // z = div u (FP64), x = p·dt ; partial Σz
 template <class T>
@@ -384,6 +422,34 @@ __global__ void __launch_bounds__(NT) k_fcg_dots(FcgV<T> f, double *part, int fi
     block_partial(a, part, 2, 0);
     block_partial(b, part, 2, 1);
 }
+// p' = s·w + βp at the cell and its neighbours; q = A p' (FP64)
+template <int D, class T>
+__device__ inline void fcg_apply(const FcgV<T> &f, const int p[3], long I, double s, double beta, double &pc,
+                                 double &qq) {
+    const Grid &g = f.g;
+    const T *__restrict__ w = f.w;
+    const T *__restrict__ L = f.L;
+    const double *__restrict__ pa = f.pa;
+    Nb n = neighbours_d<D>(g, p);
+    double lm[D], lp[D], pm[D], pp[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        lm[i] = (double)L[g.ns * i + I];
+        lp[i] = (double)L[g.ns * i + I + g.st[i]];
+        long Km = I + n.m[i], Kp = I + n.p[i];
+        pm[i] = s * (double)w[Km] + beta * pa[Km];
+        pp[i] = s * (double)w[Kp] + beta * pa[Kp];
+    }
+    pc = s * (double)w[I] + beta * pa[I];
+    double dg = 0, sl = 0, su = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dg -= (lm[i] + lp[i]);
+#pragma unroll
+    for (int i = 0; i < D; ++i) sl += pm[i] * lm[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i) su += pp[i] * lp[i];
+    qq = (pc * dg + sl) + su;
+}
 // ρ = w·r, β = -(w·q)/(p·q); p' = w + βp (on the fly at the stencil points);
 // q = A p'; partial p'·q
 template <class T>
@@ -401,24 +467,9 @@ __global__ void __launch_bounds__(NT) k_fcg_Ap(FcgV<T> f, const double *part_dot
     const Grid &g = f.g;
     double v = 0;
     COL_LOOP(f, {
-        auto pn = [&](int a0, int a1, int a2) {
-            long K = (long)map_per(g, 0, a0) + g.st[1] * map_per(g, 1, a1) + g.st[2] * map_per(g, 2, a2);
-            return s * (double)f.w[K] + beta * f.pa[K];
-        };
-        double pc = pn(p[0], p[1], p[2]);
-        double dg = 0, sl = 0, su = 0;
-        for (int i = 0; i < g.D; ++i) dg -= ((double)f.L[g.ns * i + I] + (double)f.L[g.ns * i + I + g.st[i]]);
-        for (int i = 0; i < g.D; ++i) {
-            int q[3] = {p[0], p[1], p[2]};
-            q[i] -= 1;
-            sl += pn(q[0], q[1], q[2]) * (double)f.L[g.ns * i + I];
-        }
-        for (int i = 0; i < g.D; ++i) {
-            int q[3] = {p[0], p[1], p[2]};
-            q[i] += 1;
-            su += pn(q[0], q[1], q[2]) * (double)f.L[g.ns * i + I + g.st[i]];
-        }
-        double qq = (pc * dg + sl) + su;
+        double pc, qq;
+        if (g.D == 3) fcg_apply<3>(f, p, I, s, beta, pc, qq);
+        else fcg_apply<2>(f, p, I, s, beta, pc, qq);
         f.pb[I] = pc;
         f.q[I] = qq;
         v += pc * qq;
@@ -461,4 +512,133 @@ __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, doub
     })
 }
 
+}  // namespace ia
+
+// ---------------------------------------------------------------------------
+// the coarse tail of the V-cycle in one workgroup: Vcycle!(l_t) + pcg!(l_t)
+// for every level from l_t down (levels of <= 4096 cells), so the ~20 tiny
+// launches per level become one.  Same arithmetic as the multi-block
+// kernels; dot products accumulate in FP64 then round to T.
+// ---------------------------------------------------------------------------
+namespace ia {
+constexpr int NT1 = 1024;
+constexpr int MAXLEV = 12;
+template <class T> struct TailLev {
+    Grid g;
+    const T *L, *Dg, *iD;
+    T *x, *e, *r, *z;
+    int n;  // interior cells
+};
+template <class T> struct TailArgs {
+    TailLev<T> lv[MAXLEV];
+    int l0, nlev;
+};
+__device__ inline double bsum1024(double v, double *sh) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0;
+    for (int q = 0; q < NT1 / 64; ++q) s += sh[q];
+    __syncthreads();
+    return s;  // every thread
+}
+__device__ inline void tcell(const Grid &g, int q, int p[3], long &I) {
+    int n0 = g.Ng[0] - 2, n1 = g.D > 1 ? g.Ng[1] - 2 : 1;
+    p[0] = 1 + q % n0;
+    int r = q / n0;
+    p[1] = g.D > 1 ? 1 + r % n1 : 0;
+    p[2] = g.D > 2 ? 1 + r / n1 : 0;
+    I = (long)p[0] + g.st[1] * p[1] + g.st[2] * p[2];
+}
+#define TLOOP(L, ...)                                      \
+    for (int q_ = threadIdx.x; q_ < (L).n; q_ += NT1) {    \
+        int p[3];                                          \
+        long I;                                            \
+        tcell((L).g, q_, p, I);                            \
+        __VA_ARGS__                                        \
+    }                                                      \
+    __syncthreads();
+
+template <class T> __device__ void tail_increment(const TailLev<T> &l) {
+    const T *__restrict__ e = l.e;
+    T *__restrict__ r = l.r;
+    T *__restrict__ x = l.x;
+    TLOOP(l, {
+        r[I] = r[I] - mgmult(l.g, p, I, l.L, l.Dg, e);
+        x[I] = x[I] + e[I];
+    })
+}
+template <class T> __device__ void tail_pcg(const TailLev<T> &l, int it, double *sh) {
+    T *__restrict__ e = l.e;
+    T *__restrict__ z = l.z;
+    T *__restrict__ r = l.r;
+    T *__restrict__ x = l.x;
+    const T *__restrict__ iD = l.iD;
+    double v = 0;
+    TLOOP(l, {
+        T zz = r[I] * iD[I];
+        z[I] = zz;
+        e[I] = zz;
+        v += (double)r[I] * (double)zz;
+    })
+    T rho = (T)bsum1024(v, sh);
+    if (fabs(rho) < 10 * eps_of<T>()) return;
+    for (int i = 1; i <= it; ++i) {
+        v = 0;
+        TLOOP(l, {
+            T zz = mgmult(l.g, p, I, l.L, l.Dg, (const T *)e);
+            z[I] = zz;
+            v += (double)zz * (double)e[I];
+        })
+        T alpha = rho / (T)bsum1024(v, sh);
+        bool last = i == it || fabs((double)alpha) < 1e-2;
+        v = 0;
+        TLOOP(l, {
+            x[I] += alpha * e[I];
+            T rr = r[I] - alpha * z[I];
+            r[I] = rr;
+            T zz = rr * iD[I];
+            z[I] = zz;
+            v += (double)rr * (double)zz;
+        })
+        if (last) return;
+        T rho2 = (T)bsum1024(v, sh);
+        if (fabs(rho2) < 10 * eps_of<T>()) return;
+        T beta = rho2 / rho;
+        TLOOP(l, { e[I] = beta * e[I] + z[I]; })
+        rho = rho2;
+    }
+}
+template <class T> __global__ void __launch_bounds__(NT1) k_mg_tail(TailArgs<T> a) {
+    __shared__ double sh[NT1 / 64];
+    int last = a.nlev - 1;
+    // descend: Jacobi!(l), restrict!(l -> l+1), coarse x = 0
+    for (int l = a.l0; l < last; ++l) {
+        const TailLev<T> &f = a.lv[l], &c = a.lv[l + 1];
+        TLOOP(f, { f.e[I] = f.r[I] * f.iD[I]; })
+        tail_increment(f);
+        TLOOP(c, {
+            T s = 0;
+            for (int c2 = (c.g.D > 2 ? 2 * p[2] - 1 : 0); c2 <= (c.g.D > 2 ? 2 * p[2] : 0); ++c2)
+                for (int c1 = 2 * p[1] - 1; c1 <= 2 * p[1]; ++c1)
+                    for (int c0 = 2 * p[0] - 1; c0 <= 2 * p[0]; ++c0) s += f.r[c0 + f.g.st[1] * c1 + f.g.st[2] * c2];
+            c.r[I] = s;
+            c.x[I] = T(0);
+        })
+    }
+    // ascend: pcg!(l+1), prolongate!, increment!(l)
+    for (int l = last - 1; l >= a.l0; --l) {
+        const TailLev<T> &f = a.lv[l], &c = a.lv[l + 1];
+        tail_pcg(c, 6, sh);
+        TLOOP(f, {
+            int q[3];
+            for (int d = 0; d < 3; ++d) q[d] = d < f.g.D ? (p[d] + 3) / 2 - 1 : 0;
+            f.e[I] = c.x[(long)q[0] + c.g.st[1] * q[1] + c.g.st[2] * q[2]];
+        })
+        tail_increment(f);
+    }
+    tail_pcg(a.lv[a.l0], 6, sh);
+}
 }  // namespace ia
