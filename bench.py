@@ -6,7 +6,8 @@ walls, gravity -z, density ratio 1000 (λρ = 1e-3), viscosity ratio 100
 U(0.03, 0.08)L at non-overlapping centres (numpy PCG64 seed 3, >= 2-cell
 clearance from walls, surface and each other), L = 256 cells, U = 1,
 g0 = U^2/L, ν = UL/1e4; the pressure solve is FP32 MultiLevelPoisson
-V-cycles inside FP64 iterative refinement to ||z-Ax||/||z|| <= 1e-6.
+V-cycles as the preconditioner of an FP64 flexible conjugate gradient, run to
+||z - A x||_2 / ||z||_2 <= 1e-6 (FP64 residual and pressure accumulation).
 
 A step is one full MPFMomStep! (predictor + corrector: 6 VOF sweeps, 6
 momentum sweeps, 2 viscous/RK updates, 2 projections, CFL).  Inputs are
@@ -122,22 +123,42 @@ BYTES_PER_CELL = {
 BYTES_PER_CELL["increment"] = 4 + 4 + 12 + 4 + 4 + 4 + 4
 
 
-def cpu_baseline(N_cpu, poisson, rtol, steps):
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs (copy-calibrated), when such a measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
+    """The CPU oracle (C + OpenMP, same algorithm: FP32 stencils, FP64 flexible CG
+    around the FP32 V-cycle to the same relative residual) on the same case at
+    a bounded size, timed after `warmup` untimed steps."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     bl, kw = make_case(N_cpu, poisson)
-    o = O.OracleSim((N_cpu,) * 3, (0.0, 0.0, 0.0), float(N_cpu), mg_tol=1e-4, **kw)
+    o = O.OracleSim((N_cpu,) * 3, (0.0, 0.0, 0.0), float(N_cpu), poisson="refined", poisson_rtol=rtol,
+                    smooth_it=1, **kw)
     s = pool_sdf_samples(N_cpu, bl, 0.7)
     o.lib.oc_apply_vof(o.ctx, O._ptr(s))
     o.Δt[-1] = min(o.Δt[-1], o.T.type(o.cfl()))
     O.lib(o.T).oc_set_threads(threads)
+    for _ in range(warmup):
+        o.sim_step()
     t0 = time.perf_counter()
+    its = []
     for _ in range(steps):
         o.sim_step()
+        its += [o.lib.oc_last_iters(o.ctx, 0), o.lib.oc_last_iters(o.ctx, 1)]
     t = time.perf_counter() - t0
-    return dict(value=N_cpu ** 3 * steps / t / 1e6, unit="Mcell-steps/s", cores=threads, kind="port",
-                sample=f"CPU oracle (C, OpenMP, FP32, reference WaterLily solver! tol 1e-4), same pool+bubbles "
-                       f"case at {N_cpu}^3, {steps} step(s) from rest, {t:.1f} s")
+    return dict(value=round(N_cpu ** 3 * steps / t / 1e6, 4), unit="Mcell-steps/s", cores=threads, kind="port",
+                sample=f"CPU oracle (C, OpenMP, FP32 stencils, FP64 flexible CG + FP32 V-cycle to {rtol:g}), "
+                       f"the same pool+bubbles case at {N_cpu}^3, {steps} step(s) after {warmup} warm-up, "
+                       f"{t:.1f} s, CG iterations per solve {its}")
 
 
 def main():
@@ -146,11 +167,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--N", type=int, default=256)
-    ap.add_argument("--poisson", default="refined", choices=["refined", "reference", "learned"])
+    ap.add_argument("--poisson", default="refined", choices=["refined", "plain", "reference", "learned"])
     ap.add_argument("--rtol", type=float, default=1e-6)
-    ap.add_argument("--profile", default="momentum", help="kernel class whose launches are event-timed")
-    ap.add_argument("--cpu-N", type=int, default=96)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--profile", default="increment", help="kernel class whose launches are event-timed "
+                    "(the dominant one: increment! r -= Aϵ, x += ϵ on the finest level)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the run, copy 256 MiB device-to-device once (PMC byte-counter calibration)")
+    ap.add_argument("--cpu-N", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -203,13 +227,19 @@ def main():
         "config": {"workload": f"3D two-phase pool + {len(bl)} bubbles (seed 3), {N}^3, slip walls, gravity, "
                                f"density ratio 1000, viscosity ratio 100",
                    "grid": [N, N, N], "poisson": args.poisson, "poisson_rtol": args.rtol,
-                   "vcycles_per_solve_mean": float(np.mean(iters)),
-                   "poisson_ms_per_solve_mean": round(float(np.mean(solve_ms)), 3),
+                   "cg_iterations_per_solve_mean": float(np.mean(iters)),
+                   "poisson_time_to_1e-6_ms": round(float(np.mean(solve_ms)), 3),
                    "poisson_rel_residual_max": float(np.max(rel)) if rel else None},
         "roofline": {"kernel": args.profile, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic(args.profile),
                      "bytes_per_launch": bpl, "avg_launch_ms": round(avg_ms, 4), "launches": nl.value},
     }
+    if args.calib_copy:
+        import torch
+        a = torch.empty(1 << 26, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)  # an elementwise kernel (a D2D copy would go to the DMA engine)
+        torch.cuda.synchronize()
     sim.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps)

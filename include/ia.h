@@ -39,12 +39,15 @@ enum ia_limiter { IA_LIMITER_KOREN = 0, IA_LIMITER_MINMOD = 1, IA_LIMITER_UPWIND
 /* Pressure solve.
  * REFERENCE: WaterLily solver!(ml; tol, itmx) exactly as the reference calls it
  *            (flow.jl:346): stop when Σr² <= tol (absolute) or after itmx V-cycles.
- * REFINED:   FP32 MultiLevelPoisson V-cycles inside an FP64 iterative-refinement
- *            loop; stop when ||z - A x||2 / ||z||2 <= rtol (FP64 residual, FP64
- *            pressure accumulation).  No reference counterpart (mixed precision).
+ * PLAIN:     plain multigrid to a relative residual: FP32 MultiLevelPoisson
+ *            V-cycles (+ the fine pcg!, i.e. solver!'s loop body) inside an FP64
+ *            iterative-refinement loop; stop when ||z - A x||2 / ||z||2 <= rtol.
+ * REFINED:   FP64 flexible CG (FCG(1)) preconditioned by one FP32 V-cycle (+ one
+ *            fine pcg! iteration) per iteration; same stopping test, FP64
+ *            residual and pressure accumulation.  No reference counterpart.
  * LEARNED:   REFINED, with the initial guess from the learned correction network
  *            (no reference counterpart); the FP64 residual is the acceptance test. */
-enum ia_poisson_mode { IA_POISSON_REFERENCE = 0, IA_POISSON_REFINED = 1, IA_POISSON_LEARNED = 2 };
+enum ia_poisson_mode { IA_POISSON_REFERENCE = 0, IA_POISSON_REFINED = 1, IA_POISSON_LEARNED = 2, IA_POISSON_PLAIN = 3 };
 
 /* Status codes (advection.jl:145-189 reportFillError and the glue's errors). */
 enum ia_status {

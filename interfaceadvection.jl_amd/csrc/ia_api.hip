@@ -546,6 +546,19 @@ template <class T> struct Impl {
         double rel = zn > 0 ? sqrt(rr) / zn : 0.0;
         int it = 0, k = 0;
         int par = 0;
+        while (c->P.poisson_mode == IA_POISSON_PLAIN && rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
+            // plain multigrid: one V-cycle + pcg!(fine, 6) on the unit-RMS residual, x += s·w
+            f = fview(c);
+            hipLaunchKernelGGL(k_fcg_load<T>, B, dim3(NT), 0, c->s, f, pp(c, rbuf));
+            vcycle(c, 0);
+            pcg(c, l0, 6);
+            hipLaunchKernelGGL(k_ir_accum<T>, B, dim3(NT), 0, c->s, f);
+            LAUNCHCHK(c);
+            if (int e = true_resid(c, &rr)) return e;
+            rbuf = PP_FRR;
+            rel = sqrt(rr) / zn;
+            it++;
+        }
         while (rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
             if (int e = fcg_iteration(c, k == 0 ? 1 : 0, par, rbuf)) return e;
             par ^= 1;
@@ -882,6 +895,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
         if (p->N[a] < 2) return IA_ERR_BADARG;
     if (p->eta != 0) return IA_ERR_BADARG;  // surface tension is out of scope (η == nothing)
     if (p->normal_scheme < 0 || p->normal_scheme > 1 || p->limiter < 0 || p->limiter > 2) return IA_ERR_BADARG;
+    if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PLAIN) return IA_ERR_BADARG;
     if (p->poisson_mode != IA_POISSON_REFERENCE && p->dtype != IA_F32) return IA_ERR_BADARG;
     ia_ctx *c = new ia_ctx();
     c->P = *p;

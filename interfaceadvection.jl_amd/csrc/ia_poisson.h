@@ -495,6 +495,12 @@ __global__ void __launch_bounds__(NT) k_fcg_update(FcgV<T> f, const double *part
     })
     block_partial(v, part);
 }
+// plain multigrid refinement: x += s·x_T ; r = z - A x is recomputed by the caller
+template <class T>
+__global__ void __launch_bounds__(NT) k_ir_accum(FcgV<T> f) {
+    double s = f.S->sload;
+    COL_LOOP(f, { f.x[I] += s * (double)f.w[I]; })
+}
 // u -= L ∂x ; p = x/dt ; BC!(u)  (flow.jl:331-334, :82)
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, double dtw, T ubc0, T ubc1, T ubc2) {

@@ -581,6 +581,9 @@ struct oc_ctx {
     double mg_tol;
     int mg_itmx;
     int coarse_it; /* pcg! iterations on coarse levels inside Vcycle! (WaterLily: 6) */
+    int poisson_mode, smooth_it;
+    double poisson_rtol;
+    int last_iters[2];
     char msg[512];
 };
 
@@ -1199,8 +1202,15 @@ static void proj_apply(oc_ctx *c, T dt) {
     T idt = (T)1 / dt;
     for (size_t q = 0; q < g->ns; ++q) c->p[q] = c->p[q] * idt;
 }
+int oc_project_fcg(oc_ctx *c, double dt_w, double rtol, int max_it, int smooth_it, int *iters, double *rel);
 /* flow.jl:328-335 myproject! */
 static int myproject(oc_ctx *c, T dt, int *iters, double *r2) {
+    if (c->poisson_mode == 1) {
+        double rel;
+        int e = oc_project_fcg(c, (double)dt, c->poisson_rtol, 200, c->smooth_it, iters, &rel);
+        if (r2) *r2 = rel;
+        return e;
+    }
     grid_t *g = &c->g;
     rng_t R = r_in(g);
     PFOR_R(R) { size_t I = IX(i, j, k); c->sig[I] = divu(g, c->u, I); }
@@ -1210,6 +1220,7 @@ static int myproject(oc_ctx *c, T dt, int *iters, double *r2) {
     proj_apply(c, dt);
     return OC_OK;
 }
+int oc_last_iters(oc_ctx *c, int q) { return c->last_iters[q & 1]; }
 
 /* ------------------------------------------------------------------ */
 /* MPFMomStep! (flow.jl:60-109) and advect! (advection.jl:17-78)         */
@@ -1233,7 +1244,7 @@ int oc_mom_step(oc_ctx *c, double dtd, int step_index, double *dt_next) {
     bc_vec(g, c->u, c->uBC);
     update_L(c, c->f0);
     mg_update(c);
-    myproject(c, dtc * dt, NULL, NULL);
+    myproject(c, dtc * dt, &c->last_iters[0], NULL);
     bc_vec(g, c->u, c->uBC);
     /* corrector */
     memcpy(c->f0, c->f, n * sizeof(T));
@@ -1249,7 +1260,7 @@ int oc_mom_step(oc_ctx *c, double dtd, int step_index, double *dt_next) {
     bc_vec(g, c->u, c->uBC);
     update_L(c, c->f);
     mg_update(c);
-    myproject(c, dt, NULL, NULL);
+    myproject(c, dt, &c->last_iters[1], NULL);
     bc_vec(g, c->u, c->uBC);
     T cfl = mpcfl(c);
     double nx = dmin((double)cfl, 1.2 * (double)dt);
@@ -1301,6 +1312,9 @@ oc_ctx *oc_create(const oc_params *p) {
     c->mg_tol = p->mg_tol > 0 ? p->mg_tol : 1e-4;
     c->mg_itmx = p->mg_itmx > 0 ? p->mg_itmx : 200;
     c->coarse_it = 6;
+    c->poisson_mode = p->poisson_mode;
+    c->poisson_rtol = p->poisson_rtol > 0 ? p->poisson_rtol : 1e-6;
+    c->smooth_it = p->smooth_it;
     size_t n = g->ns, nv = n * g->D;
     /* WaterLily Flow: u = uλ(i,x) = u_BC[i]; BC!; u⁰ = copy(u); μ₀ = 1; BC!(μ₀,0) */
     c->u = alloc_fill(nv, 0);
@@ -1641,13 +1655,13 @@ int oc_project_fcg(oc_ctx *c, double dt_w, double rtol, int max_it, int smooth_i
     per_bc64(g, x);
     double rn = resid64(c, z, x, r);
     double rl = zn > 0 ? rn / zn : 0.0;
-    int it = 0;
+    int it = 0, kk = 0;
     double pq = 0;
     while (rl > rtol && it < max_it) {
         precond(c, r, w, rn / sqrt((double)cnt), smooth_it);
         double wr = dot64(g, w, r);
         double beta = 0;
-        if (it > 0) beta = -dot64(g, w, q) / pq;
+        if (kk > 0) beta = -dot64(g, w, q) / pq;
         FOR_R(R) { size_t I = IX(i, j, k); p[I] = w[I] + beta * p[I]; }
         per_bc64(g, p);
         PFOR_R(R) { size_t I = IX(i, j, k); PCOORD(P); (void)P; q[I] = mult64(g, I, l0->L, p); }
@@ -1663,10 +1677,12 @@ int oc_project_fcg(oc_ctx *c, double dt_w, double rtol, int max_it, int smooth_i
         rn = sqrt(nn);
         rl = rn / zn;
         it++;
-        if (rl <= rtol) {  /* confirm with the true residual */
+        kk++;
+        if (rl <= rtol) {  /* confirm with the true residual; restart the recurrence if it drifted */
             per_bc64(g, x);
             rn = resid64(c, z, x, r);
             rl = rn / zn;
+            kk = 0;
         }
     }
     per_bc64(g, x);

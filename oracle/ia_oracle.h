@@ -41,6 +41,9 @@ typedef struct oc_params {
     double dt0;         /* initial Δt of the Simulation (WaterLily default 0.25) */
     int mg_itmx;        /* solver! itmx (reference: 200) */
     double mg_tol;      /* solver! tol on Σr² (reference: 1e-4) */
+    int poisson_mode;   /* 0: WaterLily solver! (reference); 1: FP64 flexible CG + V-cycle to poisson_rtol */
+    double poisson_rtol;
+    int smooth_it;      /* fine-level pcg! iterations per preconditioning V-cycle (mode 1) */
 } oc_params;
 
 /* field ids */

@@ -28,7 +28,8 @@ class OcParams(C.Structure):
                 ("u_bc", C.c_double * 3), ("g", C.c_double * 3), ("has_g", C.c_int),
                 ("nu", C.c_double), ("lambda_rho", C.c_double), ("lambda_mu", C.c_double),
                 ("normal_scheme", C.c_int), ("limiter", C.c_int), ("dt0", C.c_double),
-                ("mg_itmx", C.c_int), ("mg_tol", C.c_double)]
+                ("mg_itmx", C.c_int), ("mg_tol", C.c_double), ("poisson_mode", C.c_int),
+                ("poisson_rtol", C.c_double), ("smooth_it", C.c_int)]
 
 
 _libs = {}
@@ -86,6 +87,7 @@ def lib(T):
         "oc_f2face": (C.c_int, [C.c_int, ip, vp, vp, C.c_int]),
         "oc_vof_flux_face": (C.c_int, [C.c_int, ip, vp, vp, vp, vp, R, C.c_int, ip, vp, R]),
         "oc_set_threads": (None, [C.c_int]),
+        "oc_last_iters": (C.c_int, [vp, C.c_int]),
         "oc_last_error": (C.c_char_p, [vp]),
     }
     for name, (res, args) in sig.items():
@@ -149,7 +151,7 @@ class OracleSim:
 
     def __init__(self, dims, u_BC, L=1.0, *, T=np.float32, λμ=1e-2, λρ=1e-3, ν=0.0, g=None,
                  perdir=(), Δt=0.25, InterfaceSDF=None, u0=None, normal="WH", limiter="Koren",
-                 mg_tol=1e-4, mg_itmx=200, U=None):
+                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=1):
         self.T = np.dtype(T)
         self.lib = lib(self.T)
         self.dims = tuple(int(n) for n in dims)
@@ -173,6 +175,9 @@ class OracleSim:
         p.dt0 = float(Δt)
         p.mg_itmx = int(mg_itmx)
         p.mg_tol = float(mg_tol)
+        p.poisson_mode = {"reference": 0, "refined": 1}[poisson]
+        p.poisson_rtol = float(poisson_rtol)
+        p.smooth_it = int(smooth_it)
         self.ctx = self.lib.oc_create(C.byref(p))
         if not self.ctx:
             raise ValueError("oracle: bad parameters (MultiLevelPoisson needs >2 levels)")

@@ -264,6 +264,29 @@ def test_project_refined_reaches_1e6():
     g.close()
 
 
+def test_project_plain_multigrid_reaches_1e6():
+    say("plain multigrid (FP32 V-cycles in FP64 refinement), 64^2")
+    case = bubble2d(64, np.float32)
+    g, o = pair(case, poisson="plain", poisson_rtol=1e-6, poisson_max_outer=200)
+    rng = np.random.default_rng(9)
+    u = np.zeros(g.Ng + (2,), dtype=np.float32, order="F")
+    u[1:-1, 1:-1] = rng.standard_normal((64, 64, 2))
+    g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
+    o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
+    g._call("ia_update_L", 0)
+    g._call("ia_mg_update")
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    st = ia.IASolveStats()
+    g._call("ia_project", 1.0, ia.C.byref(st))
+    outer, rel = o.project_refined(1.0, 1e-6, 200)
+    assert st.rel <= 1e-6 and rel <= 1e-6
+    assert abs(st.iters - outer) <= 3
+    pg, po = O.inside(g.flow.p).astype(np.float64), O.inside(o.p).astype(np.float64)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= 1e-4 * np.abs(po).max()
+    g.close()
+
+
 # --------------------------------------------------------------------------- whole steps
 def test_quiescent_step_is_noop():
     """maintests.jl:211-220 on the GPU path."""
