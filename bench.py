@@ -171,9 +171,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--profile", default="increment", help="kernel class whose launches are event-timed "
                     "(the dominant one: increment! r -= Aϵ, x += ϵ on the finest level)")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="after the run, copy 256 MiB device-to-device once (PMC byte-counter calibration)")
-    ap.add_argument("--cpu-N", type=int, default=128)
+    ap.add_argument("--cpu-N", type=int, default=192)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -234,12 +232,6 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic(args.profile),
                      "bytes_per_launch": bpl, "avg_launch_ms": round(avg_ms, 4), "launches": nl.value},
     }
-    if args.calib_copy:
-        import torch
-        a = torch.empty(1 << 26, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.mul(a, 1.0, out=b)  # an elementwise kernel (a D2D copy would go to the DMA engine)
-        torch.cuda.synchronize()
     sim.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps)

@@ -897,6 +897,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     if (p->normal_scheme < 0 || p->normal_scheme > 1 || p->limiter < 0 || p->limiter > 2) return IA_ERR_BADARG;
     if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PLAIN) return IA_ERR_BADARG;
     if (p->poisson_mode != IA_POISSON_REFERENCE && p->dtype != IA_F32) return IA_ERR_BADARG;
+    if (p->poisson_mode == IA_POISSON_LEARNED) return IA_ERR_BADARG;  // needs a trained network: not available
     ia_ctx *c = new ia_ctx();
     c->P = *p;
     if (c->P.poisson_tol <= 0) c->P.poisson_tol = 1e-4;
