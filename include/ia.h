@@ -171,8 +171,13 @@ enum ia_prof_class {
     IA_PROF_VISC_UPDATE = 2,  /* fused viscosity + RK update + coefficients */
     IA_PROF_PCG_AE = 3,       /* pcg z = Aϵ, finest level */
     IA_PROF_PCG_UPDATE = 4,   /* pcg x,r,z update, finest level */
-    IA_PROF_INCREMENT = 5,    /* increment! r -= Aϵ, x += ϵ, finest level */
-    IA_PROF_NCLASS = 6
+    IA_PROF_INCREMENT = 5,    /* increment! r -= Aϵ, x += ϵ, finest level (unfused) */
+    IA_PROF_LOAD_INC = 6,     /* FCG residual load + Jacobi! + increment!, finest level */
+    IA_PROF_PROLONG_INC = 7,  /* prolongate! + increment! + pcg! ϵ = r·iD, finest level */
+    IA_PROF_FCG_AP = 8,       /* FCG p = w + βp, q = A p (FP64) */
+    IA_PROF_FCG_UPDATE = 9,   /* FCG x += αp, r -= αq (FP64) */
+    IA_PROF_PCG_LAST = 10,    /* last finest-level pcg! update fused with the FCG dots */
+    IA_PROF_NCLASS = 11
 };
 int ia_profile_enable(ia_ctx *c, int kernel, int on);
 int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms);

@@ -26,6 +26,7 @@ struct Level {
     Tiling t;
     CTiling ct;
     void *L = nullptr, *Dg = nullptr, *iD = nullptr, *x = nullptr, *e = nullptr, *r = nullptr, *z = nullptr;
+    void *e2 = nullptr, *z2 = nullptr;  // second ϵ/z pair of the fused pcg! iterations (levels >= 1)
     bool own = false;
 };
 
@@ -280,7 +281,7 @@ template <class T> struct Impl {
             lv.ct = ctiling(lv.g);
             size_t n = lv.g.ns;
             if (int r = alloc_zero(c, &lv.L, n * g.D * es)) return r;
-            void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z};
+            void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z, &lv.e2, &lv.z2};
             for (void **v : lb)
                 if (int r = alloc_zero(c, v, n * es)) return r;
             lv.own = true;
@@ -294,7 +295,7 @@ template <class T> struct Impl {
         if (const char *tv = getenv("IA_TAIL_LEVEL")) c->tail_level = std::max(1, atoi(tv));
         c->part_n = 2 * maxblk + 64;
         HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
-        HIPCHK(c, hipMalloc(&c->fpart, maxblk * sizeof(FillPart)));
+        HIPCHK(c, hipMalloc(&c->fpart, (maxblk + FILL_R2) * sizeof(FillPart)));
         HIPCHK(c, hipMalloc(&c->st, sizeof(StepState)));
         HIPCHK(c, hipMalloc(&c->pst, 64 * sizeof(PcgState)));
         HIPCHK(c, hipMemsetAsync(c->pst, 0, 64 * sizeof(PcgState), c->s));
@@ -353,28 +354,53 @@ template <class T> struct Impl {
         if (f0) prof_end(c, IA_PROF_INCREMENT);
     }
     static double *pp(ia_ctx *c, int k) { return c->pp + (long)k * c->pp_stride; }
-    // WaterLily pcg!(p; it)
-    static void pcg(ia_ctx *c, Level &l, int it = 6) {
+    // WaterLily pcg!(p; it).  init_done: ϵ = r·iD and the partials of ρ₀ (PP_A)
+    // were produced by the preceding fused prolongate!/increment!.  last: the
+    // final update is fused with the FCG dots of the preconditioned residual.
+    static void pcg(ia_ctx *c, Level &l, int it = 6, bool init_done = false, const FcgV<T> *last = nullptr,
+                    int first = 0) {
         LevV<T> v = view(l);
         bool f0 = &l == &c->lev[0];
         dim3 B = cb(l);
-        hipLaunchKernelGGL(k_pcg_init<T>, B, dim3(NT), 0, c->s, v, pp(c, PP_A));
+        if (!init_done) hipLaunchKernelGGL(k_pcg_init<T>, B, dim3(NT), 0, c->s, v, pp(c, PP_A));
+        // levels >= 1 fuse each ϵ update into the next z = Aϵ, alternating
+        // between the (e, z) and (e2, z2) buffer pairs
+        bool pair = l.e2 != nullptr && it > 1;
         for (int i = 1; i <= it; ++i) {
             int sa = 3 * (i - 1), su = sa + 1, se = sa + 2;
             int sin = i == 1 ? -1 : sa - 1;
-            if (f0) prof_begin(c, IA_PROF_PCG_AE);
-            hipLaunchKernelGGL(k_pcg_Ae<T>, B, dim3(NT), 0, c->s, v, c->pst, sin, sa, pp(c, PP_A), pp(c, PP_B));
-            if (f0) prof_end(c, IA_PROF_PCG_AE);
+            if (i > 1 && pair) {
+                LevV<T> w = v;
+                w.e = (i % 2 == 0) ? P<T>(l.e2) : P<T>(l.e);
+                w.z = (i % 2 == 0) ? P<T>(l.z2) : P<T>(l.z);
+                hipLaunchKernelGGL(k_pcg_epsAe<T>, B, dim3(NT), 0, c->s, v, w.e, w.z, c->pst, sa - 2, sa, pp(c, PP_C),
+                                   pp(c, PP_B));
+                v = w;
+            } else {
+                if (f0) prof_begin(c, IA_PROF_PCG_AE);
+                hipLaunchKernelGGL(k_pcg_Ae<T>, B, dim3(NT), 0, c->s, v, c->pst, sin, sa, pp(c, PP_A), pp(c, PP_B));
+                if (f0) prof_end(c, IA_PROF_PCG_AE);
+            }
+            if (i == it && last) {
+                if (f0) prof_begin(c, IA_PROF_PCG_LAST);
+                hipLaunchKernelGGL(k_pcg_last_dots<T>, B, dim3(NT), 0, c->s, v, *last, c->pst, sa, su, pp(c, PP_B),
+                                   pp(c, PP_FD), first);
+                if (f0) prof_end(c, IA_PROF_PCG_LAST);
+                continue;
+            }
             if (f0) prof_begin(c, IA_PROF_PCG_UPDATE);
             hipLaunchKernelGGL(k_pcg_update<T>, B, dim3(NT), 0, c->s, v, c->pst, sa, su, pp(c, PP_B), pp(c, PP_C), i, it);
             if (f0) prof_end(c, IA_PROF_PCG_UPDATE);
-            if (i < it) hipLaunchKernelGGL(k_pcg_eps<T>, B, dim3(NT), 0, c->s, v, c->pst, su, se, pp(c, PP_C));
+            if (i < it && !pair) hipLaunchKernelGGL(k_pcg_eps<T>, B, dim3(NT), 0, c->s, v, c->pst, su, se, pp(c, PP_C));
         }
     }
-    // Vcycle!(ml; l): the levels from c->tail_level down run in one workgroup
-    static void vcycle(ia_ctx *c, int l) {
+    // Vcycle!(ml; l); the levels from c->tail_level down run in one workgroup.
+    // pre_done: Jacobi! + increment! of level l were fused into the caller's
+    // residual load.  then_pcg: a pcg! on level l follows, so its first
+    // statement is fused into the final prolongate! + increment!.
+    static void vcycle(ia_ctx *c, int l, bool pre_done = false, bool then_pcg = false) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
-        jacobi(c, fl);
+        if (!pre_done) jacobi(c, fl);
         hipLaunchKernelGGL(k_mg_restrict<T>, cb(co), dim3(NT), 0, c->s, view(co), fl.g, P<T>(fl.r));
         if (l + 1 >= c->tail_level) {
             TailArgs<T> a;
@@ -394,11 +420,20 @@ template <class T> struct Impl {
             a.nlev = c->nlev;
             hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), 0, c->s, a);
         } else {
-            if (l + 2 < c->nlev) vcycle(c, l + 1);
-            pcg(c, co);
+            bool rec = l + 2 < c->nlev;
+            if (rec) vcycle(c, l + 1, false, true);
+            pcg(c, co, 6, rec);
         }
-        hipLaunchKernelGGL(k_mg_prolong<T>, cb(fl), dim3(NT), 0, c->s, view(fl), co.g, P<T>(co.x));
-        increment(c, fl);
+        if (then_pcg) {
+            bool f0 = l == 0;
+            if (f0) prof_begin(c, IA_PROF_PROLONG_INC);
+            hipLaunchKernelGGL(k_mg_prolong_inc_init<T>, cb(fl), dim3(NT), 0, c->s, view(fl), co.g, P<T>(co.x),
+                               pp(c, PP_A));
+            if (f0) prof_end(c, IA_PROF_PROLONG_INC);
+        } else {
+            hipLaunchKernelGGL(k_mg_prolong<T>, cb(fl), dim3(NT), 0, c->s, view(fl), co.g, P<T>(co.x));
+            increment(c, fl);
+        }
     }
     // fixed-order host sum of one partial-sum buffer (K values per block)
     static int host_total(ia_ctx *c, int buf, long nblk, int K, double *out) {
@@ -425,8 +460,8 @@ template <class T> struct Impl {
         r2 = (double)(T)r2;
         int n = 0;
         while (r2 > tol && n < itmx) {
-            vcycle(c, 0);
-            pcg(c, l0);
+            vcycle(c, 0, false, true);
+            pcg(c, l0, 6, true);
             hipLaunchKernelGGL(k_mg_center_l2<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR), ncell, 0,
                                pp(c, PP_ML2));
             LAUNCHCHK(c);
@@ -495,12 +530,22 @@ template <class T> struct Impl {
         Level &l0 = c->lev[0];
         FcgV<T> f = fview(c, par);
         dim3 B = cb(l0);
-        hipLaunchKernelGGL(k_fcg_load<T>, B, dim3(NT), 0, c->s, f, pp(c, rbuf));
-        vcycle(c, 0);
-        if (c->smooth_it > 0) pcg(c, l0, c->smooth_it);
-        hipLaunchKernelGGL(k_fcg_dots<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), first);
+        prof_begin(c, IA_PROF_LOAD_INC);
+        hipLaunchKernelGGL(k_fcg_load_inc<T>, B, dim3(NT), 0, c->s, f, view(l0), pp(c, rbuf));
+        prof_end(c, IA_PROF_LOAD_INC);
+        if (c->smooth_it > 0) {
+            vcycle(c, 0, true, true);
+            pcg(c, l0, c->smooth_it, true, &f, first);
+        } else {
+            vcycle(c, 0, true, false);
+            hipLaunchKernelGGL(k_fcg_dots<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), first);
+        }
+        prof_begin(c, IA_PROF_FCG_AP);
         hipLaunchKernelGGL(k_fcg_Ap<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), pp(c, PP_FAP), first);
+        prof_end(c, IA_PROF_FCG_AP);
+        prof_begin(c, IA_PROF_FCG_UPDATE);
         hipLaunchKernelGGL(k_fcg_update<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FAP), pp(c, PP_FU));
+        prof_end(c, IA_PROF_FCG_UPDATE);
     }
     static int fcg_iteration(ia_ctx *c, int first, int par, int rbuf) {
         if (!c->use_graphs || rbuf != (first ? PP_FRR : PP_FU)) {
@@ -549,9 +594,11 @@ template <class T> struct Impl {
         while (c->P.poisson_mode == IA_POISSON_PLAIN && rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
             // plain multigrid: one V-cycle + pcg!(fine, 6) on the unit-RMS residual, x += s·w
             f = fview(c);
-            hipLaunchKernelGGL(k_fcg_load<T>, B, dim3(NT), 0, c->s, f, pp(c, rbuf));
-            vcycle(c, 0);
-            pcg(c, l0, 6);
+            prof_begin(c, IA_PROF_LOAD_INC);
+            hipLaunchKernelGGL(k_fcg_load_inc<T>, B, dim3(NT), 0, c->s, f, view(l0), pp(c, rbuf));
+            prof_end(c, IA_PROF_LOAD_INC);
+            vcycle(c, 0, true, true);
+            pcg(c, l0, 6, true);
             hipLaunchKernelGGL(k_ir_accum<T>, B, dim3(NT), 0, c->s, f);
             LAUNCHCHK(c);
             if (int e = true_resid(c, &rr)) return e;
@@ -656,7 +703,9 @@ template <class T> struct Impl {
         prof_begin(c, IA_PROF_VOF);
         hipLaunchKernelGGL(k_vof_sweep<T>, blocks(c->t), dim3(NT), 0, c->s, a);
         prof_end(c, IA_PROF_VOF);
-        hipLaunchKernelGGL(k_fill_finalize<T>, dim3(1), dim3(NT), 0, c->s, g, c->fpart, c->t.nblk, P<T>((void *)u1),
+        hipLaunchKernelGGL(k_fill_reduce, dim3(FILL_R2), dim3(NT), 0, c->s, c->fpart, c->t.nblk, c->fpart + c->t.nblk);
+        hipLaunchKernelGGL(k_fill_finalize<T>, dim3(1), dim3(NT), 0, c->s, g, c->fpart + c->t.nblk, (long)FILL_R2,
+                           P<T>((void *)u1),
                            P<T>((void *)u2), rtol, c->st);
     }
     static void momentum(ia_ctx *c, const void *fpre, const void *u1, const void *u2, const void *uold, int d, T dt) {
@@ -942,7 +991,7 @@ void ia_destroy(ia_ctx *c) {
         for (void *b : lb)
             if (b) (void)hipFree(b);
         if (lv.own) {
-            void *ob[] = {lv.L, lv.x, lv.z};
+            void *ob[] = {lv.L, lv.x, lv.z, lv.e2, lv.z2};
             for (void *b : ob)
                 if (b) (void)hipFree(b);
         }

@@ -189,15 +189,30 @@ __global__ void __launch_bounds__(NT) k_vof_sweep(SweepArgs<T> a) {
     if (threadIdx.x == 0) a.part[blockIdx.x] = r;
 }
 
-// reportFillError (advection.jl:145-189) on the reduced partials
-template <class T>
-__global__ void __launch_bounds__(NT) k_fill_finalize(Grid g, const FillPart *part, long nblk, const T *u1,
-                                                      const T *u2, double tol, StepState *st) {
+__device__ inline FillPart fill_identity() {
     FillPart acc;
     acc.vmax = -1e300;
     acc.vmin = 1e300;
     acc.imax = acc.imin = 1L << 60;
     acc.nan = 0;
+    acc.pad = 0;
+    return acc;
+}
+// first stage of the fill reduction: FILL_R2 blocks fold the per-block
+// partials of a sweep (fill_combine is order-independent: ties go to the
+// smallest index)
+constexpr int FILL_R2 = 256;
+__global__ void __launch_bounds__(NT) k_fill_reduce(const FillPart *part, long nblk, FillPart *out) {
+    FillPart acc = fill_identity();
+    for (long b = (long)blockIdx.x * NT + threadIdx.x; b < nblk; b += (long)gridDim.x * NT) fill_combine(acc, part[b]);
+    FillPart r = block_fill(acc);
+    if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+// reportFillError (advection.jl:145-189) on the reduced partials
+template <class T>
+__global__ void __launch_bounds__(NT) k_fill_finalize(Grid g, const FillPart *part, long nblk, const T *u1,
+                                                      const T *u2, double tol, StepState *st) {
+    FillPart acc = fill_identity();
     for (long b = threadIdx.x; b < nblk; b += NT) fill_combine(acc, part[b]);
     FillPart r = block_fill(acc);
     if (threadIdx.x != 0) return;

@@ -99,28 +99,60 @@ template <int D> __device__ inline Nb neighbours_d(const Grid &g, const int p[3]
     }
     return n;
 }
-// mult(I,L,D,x)  (WaterLily: x[I]D[I] + Σ_i x[I-δi]L[I,i] + Σ_i x[I+δi]L[I+δi,i])
-template <int D, class T>
-__device__ inline T mgmult_d(const Grid &g, const int p[3], long I, const T *__restrict__ L, const T *__restrict__ Dg,
-                             const T *__restrict__ x) {
+// mult(I,L,D,x)  (WaterLily: x[I]D[I] + Σ_i x[I-δi]L[I,i] + Σ_i x[I+δi]L[I+δi,i]).
+// D[I] = -Σ_i (L[I,i] + L[I+δi,i]) is formed from the L values the stencil
+// loads anyway, in the same order as update! stores it (bit-identical, one
+// array less to read).  `xv(K)` gives x at flat index K.
+template <int D, class T, class X>
+__device__ inline T mgmult_fd(const Grid &g, const int p[3], long I, const T *__restrict__ L, X xv) {
     Nb n = neighbours_d<D>(g, p);
     T xm[D], xp[D], lm[D], lp[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-        xm[i] = x[I + n.m[i]];
-        xp[i] = x[I + n.p[i]];
+        xm[i] = xv(I + n.m[i]);
+        xp[i] = xv(I + n.p[i]);
         lm[i] = L[g.ns * i + I];
         lp[i] = L[g.ns * i + I + g.st[i]];
     }
-    T sl = 0, su = 0;
+    T dg = 0, sl = 0, su = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dg -= (lm[i] + lp[i]);
 #pragma unroll
     for (int i = 0; i < D; ++i) sl += xm[i] * lm[i];
 #pragma unroll
     for (int i = 0; i < D; ++i) su += xp[i] * lp[i];
-    return (x[I] * Dg[I] + sl) + su;
+    return (xv(I) * dg + sl) + su;
 }
-template <class T> __device__ inline T mgmult(const Grid &g, const int p[3], long I, const T *L, const T *Dg, const T *x) {
-    return g.D == 3 ? mgmult_d<3>(g, p, I, L, Dg, x) : mgmult_d<2>(g, p, I, L, Dg, x);
+// the same with x given per cell coordinate (for values formed on the fly
+// from another grid, e.g. the prolongated coarse solution)
+template <int D, class T, class X>
+__device__ inline T mgmult_fc(const Grid &g, const int p[3], long I, const T *__restrict__ L, X xv) {
+    T xm[D], xp[D], lm[D], lp[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        int q[3] = {p[0], p[1], p[2]};
+        q[i] = (g.per[i] && p[i] == 1) ? g.Ng[i] - 2 : p[i] - 1;
+        xm[i] = xv(q);
+        q[i] = (g.per[i] && p[i] == g.Ng[i] - 2) ? 1 : p[i] + 1;
+        xp[i] = xv(q);
+        lm[i] = L[g.ns * i + I];
+        lp[i] = L[g.ns * i + I + g.st[i]];
+    }
+    T dg = 0, sl = 0, su = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dg -= (lm[i] + lp[i]);
+#pragma unroll
+    for (int i = 0; i < D; ++i) sl += xm[i] * lm[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i) su += xp[i] * lp[i];
+    return (xv(p) * dg + sl) + su;
+}
+template <int D, class T>
+__device__ inline T mgmult_d(const Grid &g, const int p[3], long I, const T *__restrict__ L, const T *__restrict__ x) {
+    return mgmult_fd<D>(g, p, I, L, [&](long K) { return x[K]; });
+}
+template <class T> __device__ inline T mgmult(const Grid &g, const int p[3], long I, const T *L, const T *, const T *x) {
+    return g.D == 3 ? mgmult_d<3>(g, p, I, L, x) : mgmult_d<2>(g, p, I, L, x);
 }
 
 #define COL_LOOP(lv, ...)                                           \
@@ -133,6 +165,176 @@ template <class T> __device__ inline T mgmult(const Grid &g, const int p[3], lon
                 __VA_ARGS__                                         \
             }                                                       \
     }
+
+// Pipelined column traversal: every load of plane z+1 is issued before the
+// stores of plane z, so a wave keeps two planes of loads in flight instead
+// of waiting a full memory round trip per plane.  load(p, I) returns the
+// values a cell needs; store(p, I, values) computes and writes.
+template <class LF, class SF>
+__device__ inline void col_pipe(const Grid &g, const CTiling &t, LF load, SF store) {
+    int p[3], zlo, zhi;
+    if (!thread_col(g, t, p, zlo, zhi)) return;
+    p[2] = zlo;
+    long I = (long)p[0] + g.st[1] * p[1] + g.st[2] * zlo;
+    auto cur = load(p, I);
+    for (int z = zlo; z <= zhi; ++z) {
+        int pn[3] = {p[0], p[1], z + 1};
+        long In = I + g.st[2];
+        auto nxt = cur;
+        if (z < zhi) nxt = load(pn, In);
+        p[2] = z;
+        store(p, I, cur);
+        cur = nxt;
+        I = In;
+    }
+}
+// the operand of mult() at a cell and its neighbours, with the L values
+template <int D, class T> struct S7 {
+    T c, m[D], p[D], lm[D], lp[D];
+};
+template <int D, class T, class X>
+__device__ inline S7<D, T> ld7(const Grid &g, const int p[3], long I, const T *__restrict__ L, X xv) {
+    Nb n = neighbours_d<D>(g, p);
+    S7<D, T> s;
+    s.c = xv(I);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        s.m[i] = xv(I + n.m[i]);
+        s.p[i] = xv(I + n.p[i]);
+        s.lm[i] = L[g.ns * i + I];
+        s.lp[i] = L[g.ns * i + I + g.st[i]];
+    }
+    return s;
+}
+// mult() on loaded values, in mgmult_fd's operation order
+template <int D, class T> __device__ inline T mul7(const S7<D, T> &s) {
+    T dg = 0, sl = 0, su = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dg -= (s.lm[i] + s.lp[i]);
+#pragma unroll
+    for (int i = 0; i < D; ++i) sl += s.m[i] * s.lm[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i) su += s.p[i] * s.lp[i];
+    return (s.c * dg + sl) + su;
+}
+// run body<D>() with D the grid's dimension (2 or 3) as a compile-time constant
+#define DISPATCH_D(g, ...)                                  \
+    if ((g).D == 3) {                                       \
+        constexpr int D = 3;                                \
+        __VA_ARGS__                                         \
+    } else {                                                \
+        constexpr int D = 2;                                \
+        __VA_ARGS__                                         \
+    }
+
+// ---------------------------------------------------------------------------
+// 2.5D LDS-tiled 7-point traversal (3D only) for an operand that is formed on
+// the fly from other arrays (E(K) at flat index K).  The block's 64x4 column
+// tile walks its z range; each plane of the operand, with a one-cell halo in
+// x and y, is evaluated once into LDS (double-buffered: one barrier per
+// plane); the z-neighbours stay in registers.  Periodic axes are resolved by
+// evaluating ghost positions at their periodic image, so the tile holds
+// exactly the values the neighbour-offset stencil would read.
+//   body(p, I, c, xm, xp, ym, yp, zm, zp) runs for every interior cell.
+// ---------------------------------------------------------------------------
+__device__ inline int per_img(const Grid &g, int a, int q) {
+    if (g.per[a]) {
+        if (q == 0) return g.Ng[a] - 2;
+        if (q == g.Ng[a] - 1) return 1;
+    }
+    return q;
+}
+template <class V, class R, class LF, class CF, class PF, class BF>
+__device__ inline void tiled7(const Grid &g, const CTiling &t, LF load, CF comb, PF pre, BF body) {
+    __shared__ V tile[2][TY + 2][TX + 2];
+    long b = swizzle_block((long)blockIdx.x, t.nblk);
+    long bx = b % t.nt[0], rr = b / t.nt[0];
+    long by = rr % t.nt[1], bz = rr / t.nt[1];
+    int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    int x = 1 + (int)bx * TX + tx, y = 1 + (int)by * TY + ty;
+    int zlo = 1 + (int)bz * t.zb, zhi = min(zlo + t.zb - 1, g.Ng[2] - 2);
+    bool active = x <= g.Ng[0] - 2 && y <= g.Ng[1] - 2;
+    bool inarr = x <= g.Ng[0] - 1 && y <= g.Ng[1] - 1;
+    int xi = per_img(g, 0, x), yi = per_img(g, 1, y);
+    // halo roles: an x-halo value (edge lanes) and a y-halo value (edge rows)
+    bool hx = tx == 0 || tx == TX - 1, hy = ty == 0 || ty == TY - 1;
+    int hxq = tx == 0 ? x - 1 : x + 1, hyq = ty == 0 ? y - 1 : y + 1;
+    bool hx_ok = hx && hxq <= g.Ng[0] - 1 && y <= g.Ng[1] - 1;
+    bool hy_ok = hy && hyq <= g.Ng[1] - 1 && x <= g.Ng[0] - 1;
+    int hxc = tx == 0 ? 0 : TX + 1, hyr = ty == 0 ? 0 : TY + 1;
+    hxq = per_img(g, 0, hxq);
+    hyq = per_img(g, 1, hyq);
+    auto LD = [&](int q0, int q1, int q2) { return load((long)q0 + g.st[1] * q1 + g.st[2] * q2); };
+    R rm{}, rc{}, rp{}, rhx{}, rhy{};
+    if (inarr) {
+        rm = LD(xi, yi, per_img(g, 2, zlo - 1));
+        rc = LD(xi, yi, zlo);
+        rp = LD(xi, yi, per_img(g, 2, zlo + 1));
+    }
+    if (hx_ok) rhx = LD(hxq, yi, zlo);
+    if (hy_ok) rhy = LD(xi, hyq, zlo);
+    V em = comb(rm), ec = comb(rc);
+    long I0 = (long)x + g.st[1] * y + g.st[2] * zlo;
+    auto pv = pre(active ? I0 : 0);
+    for (int z = zlo; z <= zhi; ++z) {
+        int buf = (z - zlo) & 1;
+        long I = (long)x + g.st[1] * y + g.st[2] * z;
+        // prefetch: own column two planes ahead, halo and pre-loads one plane ahead
+        R rn{}, nhx{}, nhy{};
+        auto pvn = pv;
+        if (z < zhi) {
+            if (inarr) rn = LD(xi, yi, per_img(g, 2, z + 2));
+            if (hx_ok) nhx = LD(hxq, yi, z + 1);
+            if (hy_ok) nhy = LD(xi, hyq, z + 1);
+            pvn = pre(active ? I + g.st[2] : 0);
+        }
+        V ep = comb(rp);
+        tile[buf][ty + 1][tx + 1] = ec;
+        if (hx) tile[buf][ty + 1][hxc] = hx_ok ? comb(rhx) : V(0);
+        if (hy) tile[buf][hyr][tx + 1] = hy_ok ? comb(rhy) : V(0);
+        __syncthreads();
+        if (active) {
+            int p[3] = {x, y, z};
+            body(p, I, pv, ec, tile[buf][ty + 1][tx], tile[buf][ty + 1][tx + 2], tile[buf][ty][tx + 1],
+                 tile[buf][ty + 2][tx + 1], em, ep);
+        }
+        em = ec;
+        ec = ep;
+        rp = rn;
+        rhx = nhx;
+        rhy = nhy;
+        pv = pvn;
+    }
+}
+// the L values of a 3D cell (faces I and I+δ per axis)
+template <class V> struct L6 {
+    V m0, m1, m2, p0, p1, p2;
+};
+template <class V, class T> __device__ inline L6<V> load_L6(const Grid &g, long I, const T *__restrict__ L) {
+    L6<V> l;
+    l.m0 = (V)L[I];
+    l.m1 = (V)L[g.ns + I];
+    l.m2 = (V)L[2 * g.ns + I];
+    l.p0 = (V)L[I + 1];
+    l.p1 = (V)L[g.ns + I + g.st[1]];
+    l.p2 = (V)L[2 * g.ns + I + g.st[2]];
+    return l;
+}
+// mult() on tiled operand values, in mgmult_fd's operation order
+template <class V>
+__device__ inline V mult7(const L6<V> &l, V c, V xm, V xp, V ym, V yp, V zm, V zp) {
+    V dg = 0, sl = 0, su = 0;
+    dg -= (l.m0 + l.p0);
+    dg -= (l.m1 + l.p1);
+    dg -= (l.m2 + l.p2);
+    sl += xm * l.m0;
+    sl += ym * l.m1;
+    sl += zm * l.m2;
+    su += xp * l.p0;
+    su += yp * l.p1;
+    su += zp * l.p2;
+    return (c * dg + sl) + su;
+}
 
 // residual! part 1: r = z - A x (0 where iD == 0); partial Σr
 template <class T>
@@ -169,15 +371,34 @@ __global__ void __launch_bounds__(NT) k_mg_center_l2(LevV<T> l, const double *pa
 // Jacobi!: ϵ = r·iD
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_jacobi(LevV<T> l) {
-    COL_LOOP(l, { l.e[I] = l.r[I] * l.iD[I]; })
+    const T *__restrict__ r = l.r;
+    const T *__restrict__ iD = l.iD;
+    T *__restrict__ e = l.e;
+    struct V2 {
+        T r, d;
+    };
+    col_pipe(l.g, l.t, [&](const int *, long I) { return V2{r[I], iD[I]}; },
+             [&](const int *, long I, const V2 &v) { e[I] = v.r * v.d; });
 }
 // increment!: r -= Aϵ; x += ϵ
+template <int D, class T> __device__ inline void increment_d(const LevV<T> &l) {
+    const T *__restrict__ e = l.e;
+    T *__restrict__ r = l.r;
+    T *__restrict__ x = l.x;
+    struct V {
+        S7<D, T> s;
+        T r, x;
+    };
+    col_pipe(l.g, l.t,
+             [&](const int *p, long I) { return V{ld7<D>(l.g, p, I, l.L, [&](long K) { return e[K]; }), r[I], x[I]}; },
+             [&](const int *, long I, const V &v) {
+                 r[I] = v.r - mul7<D>(v.s);
+                 x[I] = v.x + v.s.c;
+             });
+}
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_increment(LevV<T> l) {
-    COL_LOOP(l, {
-        l.r[I] = l.r[I] - mgmult(l.g, p, I, l.L, l.Dg, l.e);
-        l.x[I] = l.x[I] + l.e[I];
-    })
+    DISPATCH_D(l.g, increment_d<D>(l);)
 }
 // restrict!: coarse r = Σ children; coarse x = 0
 template <class T>
@@ -205,6 +426,62 @@ __global__ void __launch_bounds__(NT) k_mg_prolong(LevV<T> f, Grid gc, const T *
         f.e[I] = xc[(long)q[0] + gc.st[1] * q[1] + gc.st[2] * q[2]];
     })
 }
+// prolongate! + increment! + the first statement of the pcg! that follows:
+// ϵ = coarse x[down(I)] formed on the fly at the cell and its neighbours;
+// r -= Aϵ; x += ϵ; then ϵ = r·iD and partial r·ϵ (-> ρ₀).  pcg!'s z = r·iD
+// is not stored: its first use overwrites it with Aϵ.
+template <int D, class T>
+__device__ inline void prolong_inc_init_d(const LevV<T> &l, const Grid &gc, const T *__restrict__ xc, double &v) {
+    T *__restrict__ r = l.r;
+    T *__restrict__ x = l.x;
+    T *__restrict__ e = l.e;
+    const T *__restrict__ iD = l.iD;
+    const Grid &g = l.g;
+    auto ec = [&](const int q[3]) {
+        long Kc = (long)((q[0] + 3) / 2 - 1);
+        if (D > 1) Kc += gc.st[1] * ((q[1] + 3) / 2 - 1);
+        if (D > 2) Kc += gc.st[2] * ((q[2] + 3) / 2 - 1);
+        return xc[Kc];
+    };
+    struct V {
+        S7<D, T> s;
+        T r, x, d;
+    };
+    col_pipe(
+        g, l.t,
+        [&](const int *p, long I) {
+            V w;
+            w.s.c = ec(p);
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                int q[3] = {p[0], p[1], p[2]};
+                q[i] = (g.per[i] && p[i] == 1) ? g.Ng[i] - 2 : p[i] - 1;
+                w.s.m[i] = ec(q);
+                q[i] = (g.per[i] && p[i] == g.Ng[i] - 2) ? 1 : p[i] + 1;
+                w.s.p[i] = ec(q);
+                w.s.lm[i] = l.L[g.ns * i + I];
+                w.s.lp[i] = l.L[g.ns * i + I + g.st[i]];
+            }
+            w.r = r[I];
+            w.x = x[I];
+            w.d = iD[I];
+            return w;
+        },
+        [&](const int *, long I, const V &w) {
+            T rr = w.r - mul7<D>(w.s);
+            r[I] = rr;
+            x[I] = w.x + w.s.c;
+            T zz = rr * w.d;
+            e[I] = zz;
+            v += (double)rr * (double)zz;
+        });
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_mg_prolong_inc_init(LevV<T> l, Grid gc, const T *xc, double *partA) {
+    double v = 0;
+    DISPATCH_D(l.g, prolong_inc_init_d<D>(l, gc, xc, v);)
+    block_partial(v, partA);
+}
 // pcg! (WaterLily):  z = ϵ = r·iD ; partial r·z  (-> ρ₀)
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_init(LevV<T> l, double *partA) {
@@ -216,6 +493,16 @@ __global__ void __launch_bounds__(NT) k_pcg_init(LevV<T> l, double *partA) {
         v += (double)l.r[I] * (double)zz;
     })
     block_partial(v, partA);
+}
+template <int D, class T> __device__ inline void pcg_Ae_d(const LevV<T> &l, double &v) {
+    const T *__restrict__ e = l.e;
+    T *__restrict__ z = l.z;
+    col_pipe(l.g, l.t, [&](const int *p, long I) { return ld7<D>(l.g, p, I, l.L, [&](long K) { return e[K]; }); },
+             [&](const int *, long I, const S7<D, T> &s) {
+                 T zz = mul7<D>(s);
+                 z[I] = zz;
+                 v += (double)zz * (double)s.c;
+             });
 }
 // iteration i: ρ (from ρ₀ when i == 1, else from the previous ϵ-update), z = Aϵ, partial z·ϵ
 template <class T>
@@ -234,11 +521,7 @@ __global__ void __launch_bounds__(NT) k_pcg_Ae(LevV<T> l, PcgState *S, int sin, 
     if (blockIdx.x == 0 && threadIdx.x == 0) S[sout] = st;
     if (st.done) return;
     double v = 0;
-    COL_LOOP(l, {
-        T zz = mgmult(l.g, p, I, l.L, l.Dg, l.e);
-        l.z[I] = zz;
-        v += (double)zz * (double)l.e[I];
-    })
+    DISPATCH_D(l.g, pcg_Ae_d<D>(l, v);)
     block_partial(v, partB);
 }
 // α = ρ/(z·ϵ); x += αϵ; r -= αz; z = r·iD; partial r·z; stop when i == itmax or |α| < 1e-2
@@ -255,15 +538,69 @@ __global__ void __launch_bounds__(NT) k_pcg_update(LevV<T> l, PcgState *S, int s
     st.stop = (i == itmax || fabs((double)alpha) < 1e-2) ? 1 : 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) S[sout] = st;
     double v = 0;
-    COL_LOOP(l, {
-        l.x[I] += alpha * l.e[I];
-        T rr = l.r[I] - alpha * l.z[I];
-        l.r[I] = rr;
-        T zz = rr * l.iD[I];
-        l.z[I] = zz;
-        v += (double)rr * (double)zz;
-    })
+    {
+        T *__restrict__ x = l.x;
+        T *__restrict__ r = l.r;
+        T *__restrict__ z = l.z;
+        const T *__restrict__ e = l.e;
+        const T *__restrict__ iD = l.iD;
+        struct V {
+            T x, e, r, z, d;
+        };
+        col_pipe(l.g, l.t, [&](const int *, long I) { return V{x[I], e[I], r[I], z[I], iD[I]}; },
+                 [&](const int *, long I, const V &w) {
+                     x[I] = w.x + alpha * w.e;
+                     T rr = w.r - alpha * w.z;
+                     r[I] = rr;
+                     T zz = rr * w.d;
+                     z[I] = zz;
+                     v += (double)rr * (double)zz;
+                 });
+    }
     block_partial(v, partC);
+}
+// k_pcg_eps fused with the next k_pcg_Ae: β from ρ₂ = r·z, then ϵ' = βϵ + z
+// formed on the fly at the cell and its neighbours, z' = Aϵ'; ϵ' and z' go
+// to the other buffer pair (e2, z2) so no thread overwrites what another
+// still reads; partial z'·ϵ'
+template <int D, class T>
+__device__ inline void eps_ae_d(const LevV<T> &l, T beta, T *__restrict__ e2, T *__restrict__ z2, double &v) {
+    const T *__restrict__ e = l.e;
+    const T *__restrict__ z = l.z;
+    col_pipe(l.g, l.t,
+             [&](const int *p, long I) { return ld7<D>(l.g, p, I, l.L, [&](long K) { return beta * e[K] + z[K]; }); },
+             [&](const int *, long I, const S7<D, T> &s) {
+                 T zz = mul7<D>(s);
+                 e2[I] = s.c;
+                 z2[I] = zz;
+                 v += (double)zz * (double)s.c;
+             });
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_epsAe(LevV<T> l, T *e2, T *z2, PcgState *S, int sin, int sout,
+                                                  const double *partC, double *partB) {
+    PcgState st = S[sin];
+    T beta = T(0);
+    bool go = false;
+    if (!st.done) {
+        if (st.stop) {
+            st.done = 1;
+        } else {
+            T rho2 = (T)block_total(partC, l.t.nblk);
+            if (fabs(rho2) < 10 * eps_of<T>()) {
+                st.done = 1;
+            } else {
+                beta = rho2 / (T)st.rho;
+                st.rho = (double)rho2;
+                go = true;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[sout] = st;
+    if (!go) return;
+    double v = 0;
+    DISPATCH_D(l.g, eps_ae_d<D>(l, beta, e2, z2, v);)
+    block_partial(v, partB);
 }
 // ρ₂ = r·z; exit when stop or |ρ₂| < 10eps; β = ρ₂/ρ; ϵ = βϵ + z
 template <class T>
@@ -409,6 +746,89 @@ __global__ void __launch_bounds__(NT) k_fcg_load(FcgV<T> f, const double *part_r
         f.w[I] = T(0);
     })
 }
+// k_fcg_load fused with the V-cycle's first Jacobi! + increment! on level 0:
+// ϵ = (r/s)·iD is formed on the fly at the cell and its neighbours, so
+// r_T = r/s - Aϵ and x_T = ϵ are written without storing r/s or ϵ.
+template <int D, class T>
+__device__ inline void load_inc_cell(const FcgV<T> &f, const LevV<T> &l, double s, const int p[3], long I) {
+    const double *__restrict__ r64 = f.r;
+    const T *__restrict__ iD = l.iD;
+    auto ev = [&](long K) { return (T)(r64[K] / s) * iD[K]; };
+    T wr = (T)(r64[I] / s);
+    l.r[I] = wr - mgmult_fd<D>(l.g, p, I, l.L, ev);
+    l.x[I] = T(0) + ev(I);
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_fcg_load_inc(FcgV<T> f, LevV<T> l, const double *part_rr) {
+    double s = sqrt(block_total(part_rr, f.t.nblk) / (double)f.ncell);
+    if (blockIdx.x == 0 && threadIdx.x == 0) f.S->sload = s;
+    if (l.g.D == 3) {
+        const double *__restrict__ r64 = f.r;
+        const T *__restrict__ iD = l.iD;
+        T *__restrict__ r = l.r;
+        T *__restrict__ x = l.x;
+        struct Raw {
+            double r;
+            T d;
+        };
+        struct Pre {
+            L6<T> l;
+            double rc;
+        };
+        tiled7<T, Raw>(
+            l.g, l.t, [&](long K) { return Raw{r64[K], iD[K]}; }, [&](const Raw &q) { return (T)(q.r / s) * q.d; },
+            [&](long I) { return Pre{load_L6<T>(l.g, I, l.L), r64[I]}; },
+            [&](const int *, long I, const Pre &pv, T c, T xm, T xp, T ym, T yp, T zm, T zp) {
+                T wr = (T)(pv.rc / s);
+                r[I] = wr - mult7<T>(pv.l, c, xm, xp, ym, yp, zm, zp);
+                x[I] = T(0) + c;
+            });
+    } else {
+        COL_LOOP(f, { load_inc_cell<2>(f, l, s, p, I); })
+    }
+}
+// the last pcg! iteration on level 0 inside the preconditioner, fused with
+// the FCG dots: α = ρ/(z·ϵ); x += αϵ (r and z are dead afterwards); then
+// partials of w·r and w·q with w = s·x_T
+template <class T>
+__global__ void __launch_bounds__(NT) k_pcg_last_dots(LevV<T> l, FcgV<T> f, PcgState *S, int sin, int sout,
+                                                      const double *partB, double *part, int first) {
+    PcgState st = S[sin];
+    T alpha = T(0);
+    if (!st.done) {
+        alpha = (T)st.rho / (T)block_total(partB, l.t.nblk);
+        st.alpha = (double)alpha;
+        st.stop = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[sout] = st;
+    double s = f.S->sload;
+    double a = 0, b = 0;
+    bool upd = !st.done;
+    {
+        T *__restrict__ x = l.x;
+        const T *__restrict__ e = l.e;
+        const double *__restrict__ r64 = f.r;
+        const double *__restrict__ q64 = f.q;
+        struct V {
+            double r, q;
+            T x, e;
+        };
+        col_pipe(f.g, f.t,
+                 [&](const int *, long I) { return V{r64[I], first ? 0.0 : q64[I], x[I], upd ? e[I] : T(0)}; },
+                 [&](const int *, long I, const V &w) {
+                     T xn = w.x;
+                     if (upd) {
+                         xn = xn + alpha * w.e;
+                         x[I] = xn;
+                     }
+                     double ww = s * (double)xn;
+                     a += ww * w.r;
+                     if (!first) b += ww * w.q;
+                 });
+    }
+    block_partial(a, part, 2, 0);
+    block_partial(b, part, 2, 1);
+}
 // partials of w·r and w·q (w = s·x_T)
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_dots(FcgV<T> f, double *part, int first) {
@@ -466,14 +886,35 @@ __global__ void __launch_bounds__(NT) k_fcg_Ap(FcgV<T> f, const double *part_dot
     }
     const Grid &g = f.g;
     double v = 0;
-    COL_LOOP(f, {
-        double pc, qq;
-        if (g.D == 3) fcg_apply<3>(f, p, I, s, beta, pc, qq);
-        else fcg_apply<2>(f, p, I, s, beta, pc, qq);
-        f.pb[I] = pc;
-        f.q[I] = qq;
-        v += pc * qq;
-    })
+    if (g.D == 3) {
+        const T *__restrict__ w = f.w;
+        const double *__restrict__ pa = f.pa;
+        double *__restrict__ pb = f.pb;
+        double *__restrict__ q = f.q;
+        struct Raw {
+            double pa;
+            T w;
+        };
+        tiled7<double, Raw>(
+            g, f.t, [&](long K) { return Raw{pa[K], w[K]}; },
+            [&](const Raw &q) { return s * (double)q.w + beta * q.pa; },
+            [&](long I) { return load_L6<double>(g, I, f.L); },
+            [&](const int *, long I, const L6<double> &lv, double c, double xm, double xp, double ym, double yp,
+                double zm, double zp) {
+                double qq = mult7<double>(lv, c, xm, xp, ym, yp, zm, zp);
+                pb[I] = c;
+                q[I] = qq;
+                v += c * qq;
+            });
+    } else {
+        COL_LOOP(f, {
+            double pc, qq;
+            fcg_apply<2>(f, p, I, s, beta, pc, qq);
+            f.pb[I] = pc;
+            f.q[I] = qq;
+            v += pc * qq;
+        })
+    }
     block_partial(v, part);
 }
 // α = ρ/(p·q); x += αp ; r -= αq ; partial ‖r‖²
@@ -487,12 +928,22 @@ __global__ void __launch_bounds__(NT) k_fcg_update(FcgV<T> f, const double *part
         f.S->alpha = alpha;
     }
     double v = 0;
-    COL_LOOP(f, {
-        f.x[I] += alpha * f.pb[I];
-        double rr = f.r[I] - alpha * f.q[I];
-        f.r[I] = rr;
-        v += rr * rr;
-    })
+    {
+        double *__restrict__ x = f.x;
+        double *__restrict__ r = f.r;
+        const double *__restrict__ pb = f.pb;
+        const double *__restrict__ q = f.q;
+        struct V {
+            double x, r, p, q;
+        };
+        col_pipe(f.g, f.t, [&](const int *, long I) { return V{x[I], r[I], pb[I], q[I]}; },
+                 [&](const int *, long I, const V &w) {
+                     x[I] = w.x + alpha * w.p;
+                     double rr = w.r - alpha * w.q;
+                     r[I] = rr;
+                     v += rr * rr;
+                 });
+    }
     block_partial(v, part);
 }
 // plain multigrid refinement: x += s·x_T ; r = z - A x is recomputed by the caller

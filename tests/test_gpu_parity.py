@@ -264,6 +264,36 @@ def test_project_refined_reaches_1e6():
     g.close()
 
 
+@pytest.mark.parametrize("per", [False, True])
+def test_project_refined_3d(per):
+    """3D takes the LDS-tiled solver kernels; walls and fully periodic."""
+    say("FCG + V-cycle 3D 32^3", "periodic" if per else "walls")
+    case = bubbles3d(32, np.float32)
+    if per:
+        case["kw"]["perdir"] = (1, 2, 3)
+        case["kw"]["g"] = None
+    g, o = pair(case, poisson="refined", poisson_rtol=1e-6, poisson_max_outer=100)
+    rng = np.random.default_rng(10)
+    u = np.zeros(g.Ng + (3,), dtype=np.float32, order="F")
+    u[1:-1, 1:-1, 1:-1] = rng.standard_normal((32, 32, 32, 3))
+    g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
+    o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
+    g._call("ia_update_L", 0)
+    g._call("ia_mg_update")
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    st = ia.IASolveStats()
+    g._call("ia_project", 1.0, ia.C.byref(st))
+    outer, rel = o.project_fcg(1.0, 1e-6, 100, 1)
+    assert st.rel <= 1e-6 and rel <= 1e-6
+    assert abs(st.iters - outer) <= 3
+    pg, po = O.inside(g.flow.p).astype(np.float64), O.inside(o.p).astype(np.float64)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= 1e-4 * np.abs(po).max()
+    ug, uo = g.flow.u.astype(np.float64), o.get(O.U).astype(np.float64)
+    assert maxdiff(O.inside(ug, 3), O.inside(uo, 3)) <= 1e-4 * np.abs(uo).max()
+    g.close()
+
+
 def test_project_plain_multigrid_reaches_1e6():
     say("plain multigrid (FP32 V-cycles in FP64 refinement), 64^2")
     case = bubble2d(64, np.float32)
