@@ -64,6 +64,7 @@ struct ia_ctx {
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
     int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
     int tail_level = 99; // first level handled by the one-workgroup tail kernel
+    size_t tail_lds = 0; // its dynamic LDS bytes
     hipEvent_t e0 = nullptr, e1 = nullptr;
     struct ProfClass {
         bool on = false;
@@ -289,10 +290,22 @@ template <class T> struct Impl {
             c->nlev++;
         }
         if (c->nlev <= 2) return fail(c, IA_ERR_BADARG, "MultiLevelPoisson requires size=a2^n, where n>2");
-        c->tail_level = c->nlev;
-        for (int l = 1; l < c->nlev; ++l)
-            if (interior(c->lev[l].g) <= 4096) { c->tail_level = l; break; }
-        if (const char *tv = getenv("IA_TAIL_LEVEL")) c->tail_level = std::max(1, atoi(tv));
+        // the coarsest levels run in one workgroup with every array in LDS:
+        // start at the first level of at most 4096 cells whose levels fit
+        {
+            Grid gs[MAXLEV];
+            for (int l = 0; l < c->nlev && l < MAXLEV; ++l) gs[l] = c->lev[l].g;
+            c->tail_level = c->nlev;
+            for (int l = 1; l < c->nlev && l < MAXLEV; ++l)
+                if (interior(c->lev[l].g) <= 4096 && tail_lds_bytes<T>(gs, l, c->nlev) <= TAIL_LDS_MAX) {
+                    c->tail_level = l;
+                    break;
+                }
+            c->tail_lds = c->tail_level < c->nlev ? tail_lds_bytes<T>(gs, c->tail_level, c->nlev) : 0;
+            if (c->tail_lds > 0)
+                HIPCHK(c, hipFuncSetAttribute((const void *)k_mg_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)c->tail_lds));
+        }
         c->part_n = 2 * maxblk + 64;
         HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
         HIPCHK(c, hipMalloc(&c->fpart, (maxblk + FILL_R2) * sizeof(FillPart)));
@@ -418,7 +431,7 @@ template <class T> struct Impl {
             }
             a.l0 = l + 1;
             a.nlev = c->nlev;
-            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), 0, c->s, a);
+            hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), c->tail_lds, c->s, a);
         } else {
             bool rec = l + 2 < c->nlev;
             if (rec) vcycle(c, l + 1, false, true);

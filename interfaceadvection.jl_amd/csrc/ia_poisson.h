@@ -980,6 +980,7 @@ __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, doub
 namespace ia {
 constexpr int NT1 = 1024;
 constexpr int MAXLEV = 12;
+constexpr size_t TAIL_LDS_MAX = 152 * 1024;  // of the 160 KiB of LDS per CU
 template <class T> struct TailLev {
     Grid g;
     const T *L, *Dg, *iD;
@@ -1001,101 +1002,196 @@ __device__ inline double bsum1024(double v, double *sh) {
     __syncthreads();
     return s;  // every thread
 }
-__device__ inline void tcell(const Grid &g, int q, int p[3], long &I) {
-    int n0 = g.Ng[0] - 2, n1 = g.D > 1 ? g.Ng[1] - 2 : 1;
-    p[0] = 1 + q % n0;
-    int r = q / n0;
-    p[1] = g.D > 1 ? 1 + r % n1 : 0;
-    p[2] = g.D > 2 ? 1 + r / n1 : 0;
-    I = (long)p[0] + g.st[1] * p[1] + g.st[2] * p[2];
+// ---------------------------------------------------------------------------
+// The coarsest levels in one workgroup with every array resident in LDS.
+// Each level is stored interior-only (n0 x n1 x n2, x fastest): L (one array
+// per axis: the face at the cell's low side), iD, x, ϵ, r, z.  A neighbour
+// across a non-periodic wall contributes nothing (its coefficient L is the
+// zero wall face) and is read as 0; a periodic axis wraps.  The face above
+// the last cell is the wall (0) or, periodic, the image of face 1.  The
+// arithmetic is that of the global-memory kernels, term for term.
+// ---------------------------------------------------------------------------
+template <class T> struct SLev {
+    int n[3], D, ncell;
+    bool per[3];
+    T *L[3], *iD, *x, *e, *r, *z;
+};
+template <int D, class T>
+__device__ inline T smult(const SLev<T> &s, const T *__restrict__ xv, int c, const int q[3]) {
+    T xm[D], xp[D], lm[D], lp[D];
+    int st = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        {
+            int n = s.n[a];
+            lm[a] = s.L[a][c];
+            if (q[a] > 0) xm[a] = xv[c - st];
+            else xm[a] = s.per[a] ? xv[c + (n - 1) * st] : T(0);
+            if (q[a] < n - 1) {
+                xp[a] = xv[c + st];
+                lp[a] = s.L[a][c + st];
+            } else if (s.per[a]) {
+                xp[a] = xv[c - (n - 1) * st];
+                lp[a] = s.L[a][c - (n - 1) * st];
+            } else {
+                xp[a] = T(0);
+                lp[a] = T(0);
+            }
+            st *= n;
+        }
+    }
+    T dg = 0, sl = 0, su = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) dg -= (lm[a] + lp[a]);
+#pragma unroll
+    for (int a = 0; a < D; ++a) sl += xm[a] * lm[a];
+#pragma unroll
+    for (int a = 0; a < D; ++a) su += xp[a] * lp[a];
+    return (xv[c] * dg + sl) + su;
 }
-#define TLOOP(L, ...)                                      \
-    for (int q_ = threadIdx.x; q_ < (L).n; q_ += NT1) {    \
-        int p[3];                                          \
-        long I;                                            \
-        tcell((L).g, q_, p, I);                            \
-        __VA_ARGS__                                        \
-    }                                                      \
+__device__ inline void scoord(const int n[3], int c, int q[3]) {
+    q[0] = c % n[0];
+    int r = c / n[0];
+    q[1] = r % n[1];
+    q[2] = r / n[1];
+}
+#define SLOOP(S, ...)                                        \
+    for (int c = threadIdx.x; c < (S).ncell; c += NT1) {     \
+        int q[3];                                            \
+        scoord((S).n, c, q);                                 \
+        __VA_ARGS__                                          \
+    }                                                        \
     __syncthreads();
 
-template <class T> __device__ void tail_increment(const TailLev<T> &l) {
-    const T *__restrict__ e = l.e;
-    T *__restrict__ r = l.r;
-    T *__restrict__ x = l.x;
-    TLOOP(l, {
-        r[I] = r[I] - mgmult(l.g, p, I, l.L, l.Dg, e);
-        x[I] = x[I] + e[I];
+template <int D, class T> __device__ void stail_increment(const SLev<T> &l) {
+    SLOOP(l, {
+        l.r[c] = l.r[c] - smult<D>(l, l.e, c, q);
+        l.x[c] = l.x[c] + l.e[c];
     })
 }
-template <class T> __device__ void tail_pcg(const TailLev<T> &l, int it, double *sh) {
-    T *__restrict__ e = l.e;
-    T *__restrict__ z = l.z;
-    T *__restrict__ r = l.r;
-    T *__restrict__ x = l.x;
-    const T *__restrict__ iD = l.iD;
+template <int D, class T> __device__ void stail_pcg(const SLev<T> &l, int it, double *sh) {
     double v = 0;
-    TLOOP(l, {
-        T zz = r[I] * iD[I];
-        z[I] = zz;
-        e[I] = zz;
-        v += (double)r[I] * (double)zz;
+    SLOOP(l, {
+        T zz = l.r[c] * l.iD[c];
+        l.z[c] = zz;
+        l.e[c] = zz;
+        v += (double)l.r[c] * (double)zz;
     })
     T rho = (T)bsum1024(v, sh);
     if (fabs(rho) < 10 * eps_of<T>()) return;
     for (int i = 1; i <= it; ++i) {
         v = 0;
-        TLOOP(l, {
-            T zz = mgmult(l.g, p, I, l.L, l.Dg, (const T *)e);
-            z[I] = zz;
-            v += (double)zz * (double)e[I];
+        SLOOP(l, {
+            T zz = smult<D>(l, l.e, c, q);
+            l.z[c] = zz;
+            v += (double)zz * (double)l.e[c];
         })
         T alpha = rho / (T)bsum1024(v, sh);
         bool last = i == it || fabs((double)alpha) < 1e-2;
         v = 0;
-        TLOOP(l, {
-            x[I] += alpha * e[I];
-            T rr = r[I] - alpha * z[I];
-            r[I] = rr;
-            T zz = rr * iD[I];
-            z[I] = zz;
+        SLOOP(l, {
+            l.x[c] += alpha * l.e[c];
+            T rr = l.r[c] - alpha * l.z[c];
+            l.r[c] = rr;
+            T zz = rr * l.iD[c];
+            l.z[c] = zz;
             v += (double)rr * (double)zz;
         })
         if (last) return;
         T rho2 = (T)bsum1024(v, sh);
         if (fabs(rho2) < 10 * eps_of<T>()) return;
         T beta = rho2 / rho;
-        TLOOP(l, { e[I] = beta * e[I] + z[I]; })
+        SLOOP(l, { l.e[c] = beta * l.e[c] + l.z[c]; })
         rho = rho2;
     }
 }
-template <class T> __global__ void __launch_bounds__(NT1) k_mg_tail(TailArgs<T> a) {
-    __shared__ double sh[NT1 / 64];
+// LDS bytes the levels l0..nlev-1 need
+template <class T> inline size_t tail_lds_bytes(const Grid *gs, int l0, int nlev) {
+    size_t b = 0;
+    for (int l = l0; l < nlev; ++l) {
+        size_t n = 1;
+        for (int a = 0; a < gs[l].D; ++a) n *= (size_t)(gs[l].Ng[a] - 2);
+        b += n * (gs[l].D + 5) * sizeof(T);
+    }
+    return b;
+}
+// the LDS view of tail level l (levels a.l0 .. l-1 are stored before it)
+template <class T> __device__ inline SLev<T> slev(const TailArgs<T> &a, int l, T *smem) {
+    T *base = smem;
+    for (int m = a.l0; m < l; ++m) base += (long)a.lv[m].n * (a.lv[m].g.D + 5);
+    const TailLev<T> &G = a.lv[l];
+    SLev<T> s;
+    s.D = G.g.D;
+    s.ncell = G.n;
+    for (int d = 0; d < 3; ++d) {
+        s.n[d] = d < s.D ? G.g.Ng[d] - 2 : 1;
+        s.per[d] = d < s.D && G.g.per[d];
+        s.L[d] = base + (long)d * s.ncell;
+    }
+    base += (long)s.D * s.ncell;
+    s.iD = base;
+    s.x = base + s.ncell;
+    s.e = base + 2L * s.ncell;
+    s.r = base + 3L * s.ncell;
+    s.z = base + 4L * s.ncell;
+    return s;
+}
+template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem, double *sh) {
+    for (int l = a.l0; l < a.nlev; ++l) {
+        const TailLev<T> &G = a.lv[l];
+        SLev<T> s = slev(a, l, smem);
+        // constants of the level, and the top level's restricted residual
+        SLOOP(s, {
+            long I = (long)(q[0] + 1) + G.g.st[1] * (s.D > 1 ? q[1] + 1 : 0) + G.g.st[2] * (s.D > 2 ? q[2] + 1 : 0);
+            s.L[0][c] = G.L[I];
+            s.L[1][c] = G.L[G.g.ns + I];
+            if (D > 2) s.L[2][c] = G.L[2 * G.g.ns + I];
+            s.iD[c] = G.iD[I];
+            if (l == a.l0) {
+                s.r[c] = G.r[I];
+                s.x[c] = T(0);
+            }
+        })
+    }
     int last = a.nlev - 1;
     // descend: Jacobi!(l), restrict!(l -> l+1), coarse x = 0
     for (int l = a.l0; l < last; ++l) {
-        const TailLev<T> &f = a.lv[l], &c = a.lv[l + 1];
-        TLOOP(f, { f.e[I] = f.r[I] * f.iD[I]; })
-        tail_increment(f);
-        TLOOP(c, {
+        SLev<T> f = slev(a, l, smem), cs = slev(a, l + 1, smem);
+        SLOOP(f, { f.e[c] = f.r[c] * f.iD[c]; })
+        stail_increment<D>(f);
+        SLOOP(cs, {
             T s = 0;
-            for (int c2 = (c.g.D > 2 ? 2 * p[2] - 1 : 0); c2 <= (c.g.D > 2 ? 2 * p[2] : 0); ++c2)
-                for (int c1 = 2 * p[1] - 1; c1 <= 2 * p[1]; ++c1)
-                    for (int c0 = 2 * p[0] - 1; c0 <= 2 * p[0]; ++c0) s += f.r[c0 + f.g.st[1] * c1 + f.g.st[2] * c2];
-            c.r[I] = s;
-            c.x[I] = T(0);
+            for (int c2 = (cs.D > 2 ? 2 * q[2] : 0); c2 <= (cs.D > 2 ? 2 * q[2] + 1 : 0); ++c2)
+                for (int c1 = 2 * q[1]; c1 <= 2 * q[1] + 1; ++c1)
+                    for (int c0 = 2 * q[0]; c0 <= 2 * q[0] + 1; ++c0) s += f.r[c0 + f.n[0] * (c1 + f.n[1] * c2)];
+            cs.r[c] = s;
+            cs.x[c] = T(0);
         })
     }
     // ascend: pcg!(l+1), prolongate!, increment!(l)
     for (int l = last - 1; l >= a.l0; --l) {
-        const TailLev<T> &f = a.lv[l], &c = a.lv[l + 1];
-        tail_pcg(c, 6, sh);
-        TLOOP(f, {
-            int q[3];
-            for (int d = 0; d < 3; ++d) q[d] = d < f.g.D ? (p[d] + 3) / 2 - 1 : 0;
-            f.e[I] = c.x[(long)q[0] + c.g.st[1] * q[1] + c.g.st[2] * q[2]];
+        SLev<T> f = slev(a, l, smem), cs = slev(a, l + 1, smem);
+        stail_pcg<D>(cs, 6, sh);
+        SLOOP(f, {
+            int k0 = q[0] / 2, k1 = f.D > 1 ? q[1] / 2 : 0, k2 = f.D > 2 ? q[2] / 2 : 0;
+            f.e[c] = cs.x[k0 + cs.n[0] * (k1 + cs.n[1] * k2)];
         })
-        tail_increment(f);
+        stail_increment<D>(f);
     }
-    tail_pcg(a.lv[a.l0], 6, sh);
+    SLev<T> s = slev(a, a.l0, smem);
+    stail_pcg<D>(s, 6, sh);
+    // the top level's solution back to global memory (interior)
+    const TailLev<T> &G = a.lv[a.l0];
+    SLOOP(s, {
+        long I = (long)(q[0] + 1) + G.g.st[1] * (s.D > 1 ? q[1] + 1 : 0) + G.g.st[2] * (s.D > 2 ? q[2] + 1 : 0);
+        G.x[I] = s.x[c];
+    })
+}
+template <class T> __global__ void __launch_bounds__(NT1) k_mg_tail(TailArgs<T> a) {
+    __shared__ double sh[NT1 / 64];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *smem = (T *)smem_raw;
+    if (a.lv[a.l0].g.D == 3) tail_run<3>(a, smem, sh);
+    else tail_run<2>(a, smem, sh);
 }
 }  // namespace ia
