@@ -739,8 +739,13 @@ template <class T> struct Impl {
         a.dt = dt;
         a.lr = (T)c->P.lambda_rho;
         for (int q = 0; q < 3; ++q) a.ubc[q] = (T)c->P.u_bc[q];
+        a.us = P<T>(c->nh);
+        hipLaunchKernelGGL(k_ustar<T>, blocks1(g.ns), dim3(NT), 0, c->s, a);
         prof_begin(c, IA_PROF_MOMENTUM);
-        hipLaunchKernelGGL(k_momentum<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        auto kern = k_momentum_t<3, 0, T>;
+        if (g.D == 3) kern = d == 0 ? k_momentum_t<3, 0, T> : d == 1 ? k_momentum_t<3, 1, T> : k_momentum_t<3, 2, T>;
+        else kern = d == 0 ? k_momentum_t<2, 0, T> : k_momentum_t<2, 1, T>;
+        hipLaunchKernelGGL(kern, blocks(c->t), dim3(NT), 0, c->s, a);
         prof_end(c, IA_PROF_MOMENTUM);
         std::swap(c->ru, c->ru2);
     }
@@ -793,8 +798,11 @@ template <class T> struct Impl {
         a.dt = dt;
         a.w = w;
         for (int q = 0; q < 3; ++q) a.g[q] = c->P.has_g ? (T)c->P.g[q] : T(0);
+        a.v.ff = P<T>(c->nh);
+        if (a.v.has_mu) hipLaunchKernelGGL(k_fface<T>, blocks1(g.ns), dim3(NT), 0, c->s, a.v, P<T>(c->nh));
         prof_begin(c, IA_PROF_VISC_UPDATE);
-        hipLaunchKernelGGL(k_visc_update<T>, blocks(c->t), dim3(NT), 0, c->s, a);
+        auto kv = g.D == 3 ? k_visc_update<3, T> : k_visc_update<2, T>;
+        hipLaunchKernelGGL(kv, blocks(c->t), dim3(NT), 0, c->s, a);
         prof_end(c, IA_PROF_VISC_UPDATE);
         LAUNCHCHK(c);
         return IA_OK;
@@ -891,7 +899,10 @@ template <class T> struct Impl {
         a.lr = (T)c->P.lambda_rho;
         for (int q = 0; q < 3; ++q) a.ubc[q] = (T)c->P.u_bc[q];
         a.has_mu = c->P.nu != 0;
-        hipLaunchKernelGGL(k_visc_force<T>, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->F));
+        a.ff = P<T>(c->nh);
+        hipLaunchKernelGGL(k_fface<T>, blocks1(c->g.ns), dim3(NT), 0, c->s, a, P<T>(c->nh));
+        auto kf = c->g.D == 3 ? k_visc_force<3, T> : k_visc_force<2, T>;
+        hipLaunchKernelGGL(kf, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->F));
         LAUNCHCHK(c);
         return IA_OK;
     }

@@ -259,6 +259,7 @@ template <class T> struct MomArgs {
     const int8_t *cb;
     const T *u1, *u2;    // velocity component d (dilation)
     const T *uold;       // uOld (vector)
+    T *us;               // uStar (vector, every cell incl. ghosts): the reference's n̂ alias
     int d, lim;
     T dt, lr;
     T ubc[3];
@@ -270,12 +271,6 @@ template <class T> __device__ inline T mom_ustar(const MomArgs<T> &a, int i, con
     long K = lin_vel(g, i, q);
     return a.ru_in[g.ns * i + K] / linp((a.fpre[K] + a.fpre[K - g.st[i]]) / 2, a.lr);
 }
-template <class T> __device__ inline T mom_ruf(const MomArgs<T> &a, const int q[3]) {
-    const Grid &g = a.g;
-    int d = a.d;
-    if (!g.per[d] && (q[d] == 0 || q[d] == 1 || q[d] == g.Ng[d] - 1)) return a.ubc[d];
-    return a.ruf[lin_scalar(g, q[0], q[1], q[2])];
-}
 template <class T> __device__ inline T mom_drho(const MomArgs<T> &a, int i, const int q[3]) {
     const Grid &g = a.g;
     int qq[3];
@@ -283,18 +278,36 @@ template <class T> __device__ inline T mom_drho(const MomArgs<T> &a, int i, cons
     long K = lin(g, qq);
     return (a.fpre[K] + a.fpre[K - g.st[i]]) / 2;
 }
-template <class T> __device__ inline T mom_dila(const MomArgs<T> &a, const int q[3]) {
+// uStar at every cell of the ghost-inclusive grid (flow.jl:196-206 with the
+// BC! images): computed once per sweep, read by every face that needs it
+template <class T>
+__global__ void __launch_bounds__(NT) k_ustar(MomArgs<T> a) {
+    const Grid &g = a.g;
+    long K = (long)blockIdx.x * NT + threadIdx.x;
+    if (K >= g.ns) return;
+    int q[3];
+    q[0] = (int)(K % g.Ng[0]);
+    long r = K / g.Ng[0];
+    q[1] = (int)(r % g.Ng[1]);
+    q[2] = (int)(r / g.Ng[1]);
+    for (int i = 0; i < g.D; ++i) a.us[g.ns * i + K] = mom_ustar(a, i, q);
+}
+// the sweep with the axis d and the dimension as compile-time constants
+template <int DIR, class T> __device__ inline T mom_ruf(const MomArgs<T> &a, const int q[3]) {
+    const Grid &g = a.g;
+    if (!g.per[DIR] && (q[DIR] == 0 || q[DIR] == 1 || q[DIR] == g.Ng[DIR] - 1)) return a.ubc[DIR];
+    return a.ruf[lin_scalar(g, q[0], q[1], q[2])];
+}
+template <int DIR, class T> __device__ inline T mom_dila(const MomArgs<T> &a, const int q[3]) {
     const Grid &g = a.g;
     long K = lin_scalar(g, q[0], q[1], q[2]);
-    long s = g.st[a.d];
+    long s = g.st[DIR];
     return (linp((T)a.cb[K], a.lr) * ((a.u1[K + s] - a.u1[K]) + (a.u2[K + s] - a.u2[K]))) / 2;
 }
-// ϕq (flow.jl:37-57) for face q of the staggered cell of component i
-template <class T>
+template <int DIR, class T>
 __device__ inline T mom_phiq(const MomArgs<T> &a, int i, const int q[3], T Psi, T uu, T cc, T dd) {
-    int d = a.d;
     int c[3] = {q[0], q[1], q[2]};
-    if (Psi > 0) c[d] -= 1;
+    if (Psi > 0) c[DIR] -= 1;
     T vI = cc, vd = limiter(a.lim, uu, cc, dd);
     T va = 2 * vI - vd;
     T mOut = fabs(Psi) * a.dt;
@@ -304,59 +317,52 @@ __device__ inline T mom_phiq(const MomArgs<T> &a, int i, const int q[3], T Psi, 
     T vb = l2 * va + l1 * vd;
     return (Psi * (vb + vd)) / 2;
 }
-// face flux Φ at face q (q[d] in 1..Ng-1): ϕu, ϕuL, ϕuP, ϕuR
-template <class T> __device__ inline T mom_face(const MomArgs<T> &a, int i, const int qin[3]) {
+template <int DIR, class T> __device__ inline T mom_face(const MomArgs<T> &a, int i, const int qin[3]) {
     const Grid &g = a.g;
-    int d = a.d, n = g.Ng[d];
+    int n = g.Ng[DIR];
     int q[3] = {qin[0], qin[1], qin[2]};
-    if (g.per[d] && q[d] == n - 1) q[d] = 1;  // flow.jl:241 reuses the lower-boundary flux
+    if (g.per[DIR] && q[DIR] == n - 1) q[DIR] = 1;  // flow.jl:241 reuses the lower-boundary flux
     int qm[3] = {q[0], q[1], q[2]};
     qm[i] -= 1;
-    T Psi = (mom_ruf(a, q) + mom_ruf(a, qm)) / 2;
-    auto us = [&](int off) {
-        int r[3] = {q[0], q[1], q[2]};
-        r[d] += off;
-        return mom_ustar(a, i, r);
-    };
-    if (q[d] == 1) {
-        if (!g.per[d]) {  // ϕuL
-            T c0 = us(0), cm = us(-1);
-            return Psi > 0 ? mom_phiq(a, i, q, Psi, 2 * cm - c0, cm, c0) : mom_phiq(a, i, q, Psi, us(1), c0, cm);
+    T Psi = (mom_ruf<DIR>(a, q) + mom_ruf<DIR>(a, qm)) / 2;
+    const T *__restrict__ us = a.us + g.ns * i;
+    long Q = lin(g, q), sd = g.st[DIR];
+    auto u = [&](int off) { return us[Q + off * sd]; };
+    if (q[DIR] == 1) {
+        if (!g.per[DIR]) {  // ϕuL
+            T c0 = u(0), cm = u(-1);
+            return Psi > 0 ? mom_phiq<DIR>(a, i, q, Psi, 2 * cm - c0, cm, c0) : mom_phiq<DIR>(a, i, q, Psi, u(1), c0, cm);
         }
         // ϕuP: the far-upwind value is the periodic image of I-2δ
-        int r[3] = {q[0], q[1], q[2]};
-        r[d] = n - 3;
-        return Psi > 0 ? mom_phiq(a, i, q, Psi, mom_ustar(a, i, r), us(-1), us(0))
-                       : mom_phiq(a, i, q, Psi, us(1), us(0), us(-1));
+        return Psi > 0 ? mom_phiq<DIR>(a, i, q, Psi, u(n - 4), u(-1), u(0))
+                       : mom_phiq<DIR>(a, i, q, Psi, u(1), u(0), u(-1));
     }
-    if (q[d] == n - 1) {  // ϕuR
-        T c0 = us(0), cm = us(-1);
-        return Psi < 0 ? mom_phiq(a, i, q, Psi, 2 * c0 - cm, c0, cm) : mom_phiq(a, i, q, Psi, us(-2), cm, c0);
+    if (q[DIR] == n - 1) {  // ϕuR
+        T c0 = u(0), cm = u(-1);
+        return Psi < 0 ? mom_phiq<DIR>(a, i, q, Psi, 2 * c0 - cm, c0, cm) : mom_phiq<DIR>(a, i, q, Psi, u(-2), cm, c0);
     }
-    return Psi > 0 ? mom_phiq(a, i, q, Psi, us(-2), us(-1), us(0)) : mom_phiq(a, i, q, Psi, us(1), us(0), us(-1));
+    return Psi > 0 ? mom_phiq<DIR>(a, i, q, Psi, u(-2), u(-1), u(0)) : mom_phiq<DIR>(a, i, q, Psi, u(1), u(0), u(-1));
 }
-
-template <class T>
-__global__ void __launch_bounds__(NT) k_momentum(MomArgs<T> a) {
+template <int D, int DIR, class T>
+__global__ void __launch_bounds__(NT) k_momentum_t(MomArgs<T> a) {
     const Grid &g = a.g;
     int p[3];
     if (!thread_cell(g, a.t, p)) return;
     long I = lin(g, p);
-    int d = a.d;
-    T dl0 = mom_dila(a, p);
-    for (int i = 0; i < g.D; ++i) {
+    T dl0 = mom_dila<DIR>(a, p);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
         int pu[3] = {p[0], p[1], p[2]};
-        pu[d] += 1;
-        T flo = mom_face(a, i, p), fup = mom_face(a, i, pu);
+        pu[DIR] += 1;
+        T flo = mom_face<DIR>(a, i, p), fup = mom_face<DIR>(a, i, pu);
         int pm[3] = {p[0], p[1], p[2]};
         pm[i] -= 1;
         T r = (T(0) + flo) - fup;
-        r += a.uold[g.ns * i + I] * ((dl0 + mom_dila(a, pm)) / 2);
+        r += a.uold[g.ns * i + I] * ((dl0 + mom_dila<DIR>(a, pm)) / 2);
         long o = g.ns * i + I;
         a.ru_out[o] = a.ru_in[o] + a.dt * r;
     }
 }
-
 // ---------------------------------------------------------------------------
 // viscous stress (flow.jl:113-153) fused with the RK update (flow.jl:244-252)
 // and the Poisson coefficients (flow.jl:254-259)
@@ -366,6 +372,7 @@ template <class T> struct ViscArgs {
     Tiling t;
     const T *u;       // velocity the stress is taken from (ghosts valid)
     const T *fv;      // f for the face fractions
+    const T *ff;      // the face fractions fFace (every cell incl. ghosts): the reference's n̂ alias
     T lm, mu, lr;
     T ubc[3];
     int has_mu;
@@ -378,25 +385,39 @@ template <class T> __device__ inline T visc_ff(const ViscArgs<T> &a, int c, cons
     long K = lin(g, qq);
     return (a.fv[K] + a.fv[K - g.st[c]]) / 2;
 }
+// f2face! + BCv! into n̂ (VOFutil.jl:229-234): fFace at every cell of the
+// ghost-inclusive grid, once per use
+template <class T>
+__global__ void __launch_bounds__(NT) k_fface(ViscArgs<T> a, T *ff) {
+    const Grid &g = a.g;
+    long K = (long)blockIdx.x * NT + threadIdx.x;
+    if (K >= g.ns) return;
+    int q[3];
+    q[0] = (int)(K % g.Ng[0]);
+    long r = K / g.Ng[0];
+    q[1] = (int)(r % g.Ng[1]);
+    q[2] = (int)(r / g.Ng[1]);
+    for (int c = 0; c < g.D; ++c) ff[g.ns * c + K] = visc_ff(a, c, q);
+}
 template <class T> __device__ inline T viscF(const ViscArgs<T> &a, int i, int j, const int qin[3]) {
     const Grid &g = a.g;
     int q[3] = {qin[0], qin[1], qin[2]};
     if (g.per[j] && q[j] == g.Ng[j] - 1) q[j] = 1;  // flow.jl:153
-    int qj[3] = {q[0], q[1], q[2]}, qi[3] = {q[0], q[1], q[2]};
-    qj[j] -= 1;
-    qi[i] -= 1;
-    T f1 = visc_ff(a, i, qj), f2 = visc_ff(a, i, q), f3 = visc_ff(a, j, qi), f4 = visc_ff(a, j, q);
+    long Q = lin(g, q);
+    const T *__restrict__ ffi = a.ff + g.ns * i;
+    const T *__restrict__ ffj = a.ff + g.ns * j;
+    T f1 = ffi[Q - g.st[j]], f2 = ffi[Q], f3 = ffj[Q - g.st[i]], f4 = ffj[Q];
     T s = (((f1 + f2) + f3) + f4) / 4;
     T fr = a.lr < T(1) ? tmin(tmin(tmin(f1, f2), f3), f4) : tmax(tmax(tmax(f1, f2), f3), f4);
     T mu = a.mu * tmin(linp(s, a.lm), (s > T(0.5) ? T(1) : a.lm / a.lr) * linp(fr, a.lr));
-    long Q = lin(g, q);
     const T *ui = a.u + g.ns * i, *uj = a.u + g.ns * j;
     return mu * ((ui[Q] - ui[Q - g.st[j]]) + (uj[Q] - uj[Q - g.st[i]]));
 }
-template <class T> __device__ inline T visc_force(const ViscArgs<T> &a, int i, const int p[3]) {
+template <int D, class T> __device__ inline T visc_force(const ViscArgs<T> &a, int i, const int p[3]) {
     if (!a.has_mu) return T(0);
     T r = 0;
-    for (int j = 0; j < a.g.D; ++j) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
         int pu[3] = {p[0], p[1], p[2]};
         pu[j] += 1;
         r += -viscF(a, i, j, p);
@@ -415,16 +436,17 @@ template <class T> struct UpdArgs {
     T *L;             // μ₀
     T dt, w, g[3];
 };
-template <class T>
+template <int D, class T>
 __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
     const Grid &g = a.v.g;
     int p[3];
     if (!thread_cell(g, a.v.t, p)) return;
     long I = lin(g, p);
     T wT = a.w, ac = T(1) / wT - 1;
-    for (int i = 0; i < g.D; ++i) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
         long o = g.ns * i + I;
-        T F = visc_force(a.v, i, p);
+        T F = visc_force<D>(a.v, i, p);
         T ru0 = a.u0[o] * linp((a.fcur[I] + a.fcur[I - g.st[i]]) / 2, a.v.lr);
         T run = ((ac * ru0 + a.ru[o]) + F * a.dt) * wT;
         T rho = linp((a.fupd[I] + a.fupd[I - g.st[i]]) / 2, a.v.lr);
@@ -435,13 +457,14 @@ __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
     }
 }
 // viscSurfTenρu! alone (stage entry point): Flow.f = visc
-template <class T>
+template <int D, class T>
 __global__ void __launch_bounds__(NT) k_visc_force(ViscArgs<T> a, T *F) {
     const Grid &g = a.g;
     int p[3];
     if (!thread_cell(g, a.t, p)) return;
     long I = lin(g, p);
-    for (int i = 0; i < g.D; ++i) F[g.ns * i + I] = visc_force(a, i, p);
+#pragma unroll
+    for (int i = 0; i < D; ++i) F[g.ns * i + I] = visc_force<D>(a, i, p);
 }
 // updateL! alone (stage entry point)
 template <class T>
