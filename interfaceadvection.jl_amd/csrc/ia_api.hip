@@ -63,6 +63,9 @@ struct ia_ctx {
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
     int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
+    const int *halt = nullptr;   // the flexible CG's stop flag while a pipelined solve runs
+    FcgState *pin = nullptr;     // pinned host copies of the solver state (two slots)
+    hipEvent_t it_ev[2] = {nullptr, nullptr};
     int tail_level = 99; // first level handled by the one-workgroup tail kernel
     size_t tail_lds = 0; // its dynamic LDS bytes
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -211,6 +214,8 @@ __global__ void __launch_bounds__(NT) k_bcvel(Grid g, Tiling t, T *a, T d0, T d1
 // ---------------------------------------------------------------------------
 // typed implementation
 // ---------------------------------------------------------------------------
+// the stop flag that view() hands to the kernels of the pipelined solve in progress
+static thread_local const int *cur_halt = nullptr;
 template <class T> struct Impl {
     static double epsT() { return sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16; }
 
@@ -314,6 +319,8 @@ template <class T> struct Impl {
         HIPCHK(c, hipMemsetAsync(c->pst, 0, 64 * sizeof(PcgState), c->s));
         HIPCHK(c, hipMalloc(&c->fst, sizeof(FcgState)));
         HIPCHK(c, hipMemsetAsync(c->fst, 0, sizeof(FcgState), c->s));
+        HIPCHK(c, hipHostMalloc((void **)&c->pin, 2 * sizeof(FcgState)));
+        for (auto &ev : c->it_ev) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         {
             long mb = 0;
             for (int l = 0; l < c->nlev; ++l) mb = std::max(mb, c->lev[l].ct.nblk);
@@ -344,6 +351,7 @@ template <class T> struct Impl {
     }
     static LevV<T> view(Level &l) {
         LevV<T> v;
+        v.halt = cur_halt;
         v.g = l.g;
         v.t = l.ct;
         v.L = P<T>(l.L);
@@ -431,6 +439,7 @@ template <class T> struct Impl {
             }
             a.l0 = l + 1;
             a.nlev = c->nlev;
+            a.halt = cur_halt;
             hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), c->tail_lds, c->s, a);
         } else {
             bool rec = l + 2 < c->nlev;
@@ -594,6 +603,7 @@ template <class T> struct Impl {
         }
         FcgV<T> f = fview(c);
         dim3 B = cb(l0);
+        hipLaunchKernelGGL(k_fcg_halt, dim3(1), dim3(64), 0, c->s, c->fst, 0);
         hipLaunchKernelGGL(k_fcg_init<T>, B, dim3(NT), 0, c->s, f, P<T>(u), P<T>(c->p), dtw, pp(c, PP_FI));
         hipLaunchKernelGGL(k_fcg_center_z<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FI), pp(c, PP_FZ));
         double zn2 = 0, rr = 0;
@@ -602,7 +612,7 @@ template <class T> struct Impl {
         int rbuf = PP_FRR;  // buffer holding the partials of ‖r‖²
         double zn = sqrt(zn2);
         double rel = zn > 0 ? sqrt(rr) / zn : 0.0;
-        int it = 0, k = 0;
+        int it = 0;
         int par = 0;
         while (c->P.poisson_mode == IA_POISSON_PLAIN && rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
             // plain multigrid: one V-cycle + pcg!(fine, 6) on the unit-RMS residual, x += s·w
@@ -619,19 +629,55 @@ template <class T> struct Impl {
             rel = sqrt(rr) / zn;
             it++;
         }
-        while (rel > c->P.poisson_rtol && it < c->P.poisson_max_outer) {
-            if (int e = fcg_iteration(c, k == 0 ? 1 : 0, par, rbuf)) return e;
-            par ^= 1;
-            rbuf = PP_FU;
-            if (int e = host_total(c, PP_FU, nb, 1, &rr)) return e;
-            rel = sqrt(rr) / zn;
-            it++;
-            k++;
-            if (rel <= c->P.poisson_rtol) {  // confirm with the true FP64 residual; restart if it drifted
-                if (int e = true_resid(c, &rr)) return e;
-                rbuf = PP_FRR;
+        // flexible CG, pipelined: the host queues iteration n+1 before it reads
+        // iteration n's residual, so the device never waits for the host; the
+        // device-side test sets a stop flag that turns a queued iteration past
+        // convergence into immediate returns
+        const double rtol = c->P.poisson_rtol;
+        const int maxo = c->P.poisson_max_outer;
+        if (rel > rtol && it < maxo) {
+            struct HaltScope {
+                explicit HaltScope(const int *h) { cur_halt = h; }
+                ~HaltScope() { cur_halt = nullptr; }
+            } hs(&c->fst->halt);
+            int queued = 0, qslot = 0, rslot = 0;
+            auto enqueue = [&](int first) -> int {
+                if (int e = fcg_iteration(c, first, par, rbuf)) return e;
+                par ^= 1;
+                rbuf = PP_FU;
+                hipLaunchKernelGGL(k_fcg_check<T>, dim3(1), dim3(NT), 0, c->s, fview(c), pp(c, PP_FU), zn, rtol);
+                HIPCHK(c, hipMemcpyAsync(&c->pin[qslot], c->fst, sizeof(FcgState), hipMemcpyDeviceToHost, c->s));
+                HIPCHK(c, hipEventRecord(c->it_ev[qslot], c->s));
+                qslot ^= 1;
+                queued++;
+                return IA_OK;
+            };
+            if (int e = enqueue(1)) return e;
+            for (;;) {
+                if (queued < 2 && it + queued < maxo)
+                    if (int e = enqueue(0)) return e;
+                HIPCHK(c, hipEventSynchronize(c->it_ev[rslot]));
+                FcgState st = c->pin[rslot];
+                rslot ^= 1;
+                queued--;
+                it++;
+                rr = st.rr;
                 rel = sqrt(rr) / zn;
-                k = 0;
+                if (st.halt) {
+                    // the queued iteration (if any) returned at once; confirm with
+                    // the true FP64 residual and restart from it if it drifted
+                    HIPCHK(c, hipStreamSynchronize(c->s));
+                    queued = 0;
+                    rslot = qslot;
+                    if (int e = true_resid(c, &rr)) return e;
+                    rel = sqrt(rr) / zn;
+                    if (rel <= rtol || it >= maxo) break;
+                    hipLaunchKernelGGL(k_fcg_halt, dim3(1), dim3(64), 0, c->s, c->fst, 0);
+                    rbuf = PP_FRR;
+                    if (int e = enqueue(1)) return e;
+                    continue;
+                }
+                if (queued == 0) break;  // iteration cap reached
             }
         }
         f = fview(c);
@@ -1022,6 +1068,9 @@ void ia_destroy(ia_ctx *c) {
     }
     for (auto &p : c->prof)
         for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->it_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->pin) (void)hipHostFree(c->pin);
     for (auto &ex : c->fcg_graph)
         if (ex) (void)hipGraphExecDestroy(ex);
     if (c->e0) (void)hipEventDestroy(c->e0);

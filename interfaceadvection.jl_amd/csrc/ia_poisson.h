@@ -50,6 +50,9 @@ struct PcgState {
 // scalar state of the flexible CG
 struct FcgState {
     double sload, wr, beta, pq, alpha;
+    double rr;      // ‖r‖² after the last accepted update
+    int halt, pad;  // set when the recursive residual met the tolerance:
+                    // every solver kernel queued after that returns at once
 };
 
 // every block of a consumer sums the producer's per-block partials itself
@@ -76,7 +79,12 @@ template <class T> struct LevV {
     CTiling t;
     const T *L, *Dg, *iD;
     T *x, *e, *r, *z;
+    const int *halt;  // nullptr, or the flexible CG's stop flag
 };
+// a kernel of a queued-ahead solver iteration returns at once when the
+// solve has already converged (uniform for the whole grid: before any barrier)
+#define HALT_IF(l)                      \
+    if ((l).halt && *(l).halt) return;
 
 // neighbour offsets of an interior cell: ±δ on every axis, wrapped on a
 // periodic axis at the boundary (perBC! on read); ghosts of non-periodic
@@ -371,6 +379,7 @@ __global__ void __launch_bounds__(NT) k_mg_center_l2(LevV<T> l, const double *pa
 // Jacobi!: ϵ = r·iD
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_jacobi(LevV<T> l) {
+    HALT_IF(l)
     const T *__restrict__ r = l.r;
     const T *__restrict__ iD = l.iD;
     T *__restrict__ e = l.e;
@@ -398,11 +407,13 @@ template <int D, class T> __device__ inline void increment_d(const LevV<T> &l) {
 }
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_increment(LevV<T> l) {
+    HALT_IF(l)
     DISPATCH_D(l.g, increment_d<D>(l);)
 }
 // restrict!: coarse r = Σ children; coarse x = 0
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_restrict(LevV<T> c, Grid gf, const T *rf) {
+    HALT_IF(c)
     COL_LOOP(c, {
         int lo[3], hi[3];
         for (int a = 0; a < 3; ++a) {
@@ -420,6 +431,7 @@ __global__ void __launch_bounds__(NT) k_mg_restrict(LevV<T> c, Grid gf, const T 
 // prolongate!: fine ϵ = coarse x[down(I)]
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_prolong(LevV<T> f, Grid gc, const T *xc) {
+    HALT_IF(f)
     COL_LOOP(f, {
         int q[3];
         for (int a = 0; a < 3; ++a) q[a] = a < f.g.D ? (p[a] + 3) / 2 - 1 : 0;
@@ -478,6 +490,7 @@ __device__ inline void prolong_inc_init_d(const LevV<T> &l, const Grid &gc, cons
 }
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_prolong_inc_init(LevV<T> l, Grid gc, const T *xc, double *partA) {
+    HALT_IF(l)
     double v = 0;
     DISPATCH_D(l.g, prolong_inc_init_d<D>(l, gc, xc, v);)
     block_partial(v, partA);
@@ -485,6 +498,7 @@ __global__ void __launch_bounds__(NT) k_mg_prolong_inc_init(LevV<T> l, Grid gc, 
 // pcg! (WaterLily):  z = ϵ = r·iD ; partial r·z  (-> ρ₀)
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_init(LevV<T> l, double *partA) {
+    HALT_IF(l)
     double v = 0;
     COL_LOOP(l, {
         T zz = l.r[I] * l.iD[I];
@@ -508,6 +522,7 @@ template <int D, class T> __device__ inline void pcg_Ae_d(const LevV<T> &l, doub
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_Ae(LevV<T> l, PcgState *S, int sin, int sout, const double *partA,
                                                double *partB) {
+    HALT_IF(l)
     PcgState st;
     if (sin < 0) {
         T rho = (T)block_total(partA, l.t.nblk);
@@ -528,6 +543,7 @@ __global__ void __launch_bounds__(NT) k_pcg_Ae(LevV<T> l, PcgState *S, int sin, 
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_update(LevV<T> l, PcgState *S, int sin, int sout, const double *partB,
                                                    double *partC, int i, int itmax) {
+    HALT_IF(l)
     PcgState st = S[sin];
     if (st.done) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S[sout] = st;
@@ -579,6 +595,7 @@ __device__ inline void eps_ae_d(const LevV<T> &l, T beta, T *__restrict__ e2, T 
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_epsAe(LevV<T> l, T *e2, T *z2, PcgState *S, int sin, int sout,
                                                   const double *partC, double *partB) {
+    HALT_IF(l)
     PcgState st = S[sin];
     T beta = T(0);
     bool go = false;
@@ -605,6 +622,7 @@ __global__ void __launch_bounds__(NT) k_pcg_epsAe(LevV<T> l, T *e2, T *z2, PcgSt
 // ρ₂ = r·z; exit when stop or |ρ₂| < 10eps; β = ρ₂/ρ; ϵ = βϵ + z
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_eps(LevV<T> l, PcgState *S, int sin, int sout, const double *partC) {
+    HALT_IF(l)
     PcgState st = S[sin];
     if (!st.done) {
         if (st.stop) {
@@ -760,6 +778,7 @@ __device__ inline void load_inc_cell(const FcgV<T> &f, const LevV<T> &l, double 
 }
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_load_inc(FcgV<T> f, LevV<T> l, const double *part_rr) {
+    HALT_IF(l)
     double s = sqrt(block_total(part_rr, f.t.nblk) / (double)f.ncell);
     if (blockIdx.x == 0 && threadIdx.x == 0) f.S->sload = s;
     if (l.g.D == 3) {
@@ -793,6 +812,7 @@ __global__ void __launch_bounds__(NT) k_fcg_load_inc(FcgV<T> f, LevV<T> l, const
 template <class T>
 __global__ void __launch_bounds__(NT) k_pcg_last_dots(LevV<T> l, FcgV<T> f, PcgState *S, int sin, int sout,
                                                       const double *partB, double *part, int first) {
+    HALT_IF(l)
     PcgState st = S[sin];
     T alpha = T(0);
     if (!st.done) {
@@ -832,6 +852,7 @@ __global__ void __launch_bounds__(NT) k_pcg_last_dots(LevV<T> l, FcgV<T> f, PcgS
 // partials of w·r and w·q (w = s·x_T)
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_dots(FcgV<T> f, double *part, int first) {
+    if (f.S->halt) return;
     double a = 0, b = 0;
     double s = f.S->sload;
     COL_LOOP(f, {
@@ -874,6 +895,7 @@ __device__ inline void fcg_apply(const FcgV<T> &f, const int p[3], long I, doubl
 // q = A p'; partial p'·q
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_Ap(FcgV<T> f, const double *part_dots, double *part, int first) {
+    if (f.S->halt) return;
     double wr = block_total(part_dots, f.t.nblk, 2, 0);
     double wq = first ? 0.0 : block_total(part_dots, f.t.nblk, 2, 1);
     FcgState st = *f.S;
@@ -920,6 +942,7 @@ __global__ void __launch_bounds__(NT) k_fcg_Ap(FcgV<T> f, const double *part_dot
 // α = ρ/(p·q); x += αp ; r -= αq ; partial ‖r‖²
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_update(FcgV<T> f, const double *part_pq, double *part) {
+    if (f.S->halt) return;
     double pq = block_total(part_pq, f.t.nblk);
     double alpha = f.S->wr / pq;
     __syncthreads();
@@ -945,6 +968,20 @@ __global__ void __launch_bounds__(NT) k_fcg_update(FcgV<T> f, const double *part
                  });
     }
     block_partial(v, part);
+}
+// convergence test of the recursive residual on the device, so the host can
+// queue the next iteration before it reads this one's result
+template <class T>
+__global__ void __launch_bounds__(NT) k_fcg_check(FcgV<T> f, const double *part_rr, double zn, double rtol) {
+    if (f.S->halt) return;
+    double rr = block_total(part_rr, f.t.nblk);
+    if (threadIdx.x == 0) {
+        f.S->rr = rr;
+        if (sqrt(rr) / zn <= rtol) f.S->halt = 1;
+    }
+}
+__global__ void k_fcg_halt(FcgState *S, int v) {
+    if (threadIdx.x == 0) S->halt = v;
 }
 // plain multigrid refinement: x += s·x_T ; r = z - A x is recomputed by the caller
 template <class T>
@@ -990,6 +1027,7 @@ template <class T> struct TailLev {
 template <class T> struct TailArgs {
     TailLev<T> lv[MAXLEV];
     int l0, nlev;
+    const int *halt;
 };
 __device__ inline double bsum1024(double v, double *sh) {
     v = wave_sum(v);
@@ -1188,6 +1226,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
     })
 }
 template <class T> __global__ void __launch_bounds__(NT1) k_mg_tail(TailArgs<T> a) {
+    HALT_IF(a)
     __shared__ double sh[NT1 / 64];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = (T *)smem_raw;
