@@ -45,8 +45,10 @@ enum ia_limiter { IA_LIMITER_KOREN = 0, IA_LIMITER_MINMOD = 1, IA_LIMITER_UPWIND
  * REFINED:   FP64 flexible CG (FCG(1)) preconditioned by one FP32 V-cycle (+ one
  *            fine pcg! iteration) per iteration; same stopping test, FP64
  *            residual and pressure accumulation.  No reference counterpart.
- * LEARNED:   REFINED, with the initial guess from the learned correction network
- *            (no reference counterpart); the FP64 residual is the acceptance test. */
+ * LEARNED:   REFINED, whose warm start is first corrected by the learned network
+ *            (ia_learned_*; no reference counterpart).  The correction is kept
+ *            only when it lowers the FP64 residual.  3D, FP32, N[0] % 16 == 0,
+ *            N[1] and N[2] even. */
 enum ia_poisson_mode { IA_POISSON_REFERENCE = 0, IA_POISSON_REFINED = 1, IA_POISSON_LEARNED = 2, IA_POISSON_PLAIN = 3 };
 
 /* Status codes (advection.jl:145-189 reportFillError and the glue's errors). */
@@ -161,6 +163,39 @@ int ia_update_L(ia_ctx *c, int use_f0);
 int ia_mg_update(ia_ctx *c);
 /* myproject!(flow,pois,w) with dt_w = w*Δt, then BC!(u)  flow.jl:328-347, :82 */
 int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats);
+
+/* Learned Poisson correction (no counterpart in the reference).  A five-layer
+ * 3x3x3 convolutional network (3 -> 16 -> 16 -> 16 -> 16 -> 2 channels, ReLU
+ * between layers), evaluated with BF16 matrix-core products and FP32
+ * accumulation.  Inputs per cell: f, z/rms(z), r0/rms(r0) with r0 = z - A x0
+ * the residual of the warm start; the warm start becomes
+ * x0 + iD·(2/3 r0 + rms(r0)·(out0 + 2x2x2-block-average(out1))), a damped
+ * Jacobi step plus the network's correction in the same scaled units.  Trained on the device
+ * from the solver's own operator: loss ‖z - A x‖² / ‖r0‖², one Adam step per
+ * projection while training is on.
+ * Parameters: IA_LEARNED_NPAR floats, W[layer][tap][ci][co] (tap =
+ * (dz+1)*9 + (dy+1)*3 + dx+1, 16x16 channels per tap) for the 5 layers, then
+ * the 5 bias vectors of 16.  Input channels >= 3 of layer 0 and output
+ * channels >= 2 of layer 4 are unused and stay zero. */
+#define IA_LEARNED_NPAR (5 * (27 * 16 * 16 + 16))
+typedef struct ia_learned_stats {
+    int projections;       /* projections that evaluated the network */
+    int accepted;          /* ... whose correction lowered the residual */
+    int train_steps;       /* Adam steps taken */
+    double last_ratio;     /* ‖r‖² after / before the last correction */
+} ia_learned_info;
+/* He-normal initialisation from `seed` (3D FP32 contexts). */
+int ia_learned_init(ia_ctx *c, unsigned seed);
+int ia_learned_set_weights(ia_ctx *c, const float *w, size_t n);
+int ia_learned_get_weights(ia_ctx *c, float *w, size_t n);
+/* Train (one Adam step per LEARNED projection) while on != 0. */
+int ia_learned_train(ia_ctx *c, int on, double lr);
+int ia_learned_stats(ia_ctx *c, ia_learned_info *s);
+/* The network alone on host features (3 planes of the interior, x fastest)
+ * -> 2 output planes; and the parameter gradient for a given output
+ * gradient (IA_LEARNED_NPAR floats).  For numerical checks. */
+int ia_learned_forward(ia_ctx *c, const float *feat, float *out);
+int ia_learned_grad(ia_ctx *c, const float *feat, const float *dout, float *gw);
 
 /* Kernel-class timing with HIP events on the context's stream (no reference
  * counterpart; used by the benchmark to measure one kernel's launches inside

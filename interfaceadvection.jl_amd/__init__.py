@@ -46,6 +46,14 @@ class IAStepStats(C.Structure):
                 ("solve", IASolveStats * 2)]
 
 
+class IALearnedInfo(C.Structure):
+    _fields_ = [("projections", C.c_int), ("accepted", C.c_int), ("train_steps", C.c_int),
+                ("last_ratio", C.c_double)]
+
+
+# parameters of the learned correction: 5 layers x (27 taps x 16 x 16) weights, then 5 x 16 biases
+LEARNED_NPAR = 5 * (27 * 16 * 16 + 16)
+
 _lib = None
 
 
@@ -80,6 +88,13 @@ def lib():
         "ia_profile_enable": (C.c_int, [vp, C.c_int, C.c_int]),
         "ia_profile_read": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "ia_last_error": (C.c_char_p, [vp]),
+        "ia_learned_init": (C.c_int, [vp, C.c_uint]),
+        "ia_learned_set_weights": (C.c_int, [vp, vp, sz]),
+        "ia_learned_get_weights": (C.c_int, [vp, vp, sz]),
+        "ia_learned_train": (C.c_int, [vp, C.c_int, C.c_double]),
+        "ia_learned_stats": (C.c_int, [vp, C.POINTER(IALearnedInfo)]),
+        "ia_learned_forward": (C.c_int, [vp, vp, vp]),
+        "ia_learned_grad": (C.c_int, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -311,6 +326,50 @@ class TwoPhaseSimulation:
     def sim_time(self):
         """WaterLily sim_time: Σ Δt[1:end-1] scaled by U/L is the caller's choice; raw time here."""
         return float(sum(self.flow.Δt[:-1]))
+
+    # ---- learned Poisson correction (no reference counterpart)
+    def learned_init(self, seed=0):
+        """He-normal initialisation of the correction network."""
+        self._call("ia_learned_init", int(seed))
+
+    def learned_train(self, on=True, lr=1e-3):
+        """One Adam step per LEARNED projection while on."""
+        self._call("ia_learned_train", int(bool(on)), float(lr))
+
+    @property
+    def learned_weights(self):
+        w = np.empty(LEARNED_NPAR, dtype=np.float32)
+        self._call("ia_learned_get_weights", _ptr(w), w.size)
+        return w
+
+    @learned_weights.setter
+    def learned_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._call("ia_learned_set_weights", _ptr(w), w.size)
+
+    def learned_stats(self):
+        s = IALearnedInfo()
+        self._call("ia_learned_stats", C.byref(s))
+        return dict(projections=s.projections, accepted=s.accepted, train_steps=s.train_steps,
+                    last_ratio=s.last_ratio)
+
+    def learned_forward(self, feat):
+        """The network on features of shape (3, N0, N1, N2) -> (2, N0, N1, N2)."""
+        N = tuple(n - 2 for n in self.Ng)
+        src = np.stack([np.asarray(feat[c], dtype=np.float32).reshape(-1, order="F") for c in range(3)])
+        out = np.empty((2, int(np.prod(N))), dtype=np.float32)
+        self._call("ia_learned_forward", _ptr(np.ascontiguousarray(src)), _ptr(out))
+        return np.stack([out[c].reshape(N, order="F") for c in range(2)])
+
+    def learned_grad(self, feat, dout):
+        """Parameter gradient of Σ dout·net(feat) (LEARNED_NPAR values)."""
+        src = np.ascontiguousarray(np.stack([np.asarray(feat[c], dtype=np.float32).reshape(-1, order="F")
+                                             for c in range(3)]))
+        d = np.ascontiguousarray(np.stack([np.asarray(dout[c], dtype=np.float32).reshape(-1, order="F")
+                                           for c in range(2)]))
+        gw = np.empty(LEARNED_NPAR, dtype=np.float32)
+        self._call("ia_learned_grad", _ptr(src), _ptr(d), _ptr(gw))
+        return gw
 
     def mg_levels(self):
         n = self._lib.ia_mg_levels(self._ctx, 0, None)

@@ -6,6 +6,7 @@
 // and fill flags).
 #include "ia.h"
 #include "ia_poisson.h"
+#include "ia_learned.h"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -63,6 +64,21 @@ struct ia_ctx {
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
     int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
+    struct Cnn {
+        bool ready = false;
+        void *act[CNN_L + 1] = {};   // layer inputs/outputs, BF16 channels-last
+        void *grad[2] = {};          // output-gradient ping-pong
+        double *xkeep = nullptr;
+        float *gtmp = nullptr, *io = nullptr;
+        float *w = nullptr, *m = nullptr, *v = nullptr, *gw = nullptr, *zbias = nullptr, *wpart = nullptr;
+        void *ff = nullptr, *fb = nullptr;  // MFMA operand order, [layer][27][64] x 4 bf16
+        long wnblk = 0;
+        int train = 0;
+        double lr = 1e-3;
+        long step = 0;
+        int projections = 0, accepted = 0;
+        double last_ratio = 0;
+    } cnn;
     const int *halt = nullptr;   // the flexible CG's stop flag while a pipelined solve runs
     FcgState *pin = nullptr;     // pinned host copies of the solver state (two slots)
     hipEvent_t it_ev[2] = {nullptr, nullptr};
@@ -609,6 +625,8 @@ template <class T> struct Impl {
         double zn2 = 0, rr = 0;
         if (int e = host_total(c, PP_FZ, nb, 1, &zn2)) return e;
         if (int e = true_resid(c, &rr)) return e;
+        if (c->P.poisson_mode == IA_POISSON_LEARNED)
+            if (int e = cnn_prelude(c, &rr)) return e;
         int rbuf = PP_FRR;  // buffer holding the partials of ‖r‖²
         double zn = sqrt(zn2);
         double rel = zn > 0 ? sqrt(rr) / zn : 0.0;
@@ -690,6 +708,187 @@ template <class T> struct Impl {
             ss->r2 = rr;
             ss->rel = rel;
         }
+        return IA_OK;
+    }
+    // ----------------------------------------------------------- learned correction
+    static CnnTiles cnn_tiles(ia_ctx *c) {
+        CnnTiles t;
+        t.g = c->lev[0].g;
+        t.tx = (t.g.Ng[0] - 2) / 16;
+        t.ntile = (long)t.tx * (t.g.Ng[1] - 2) * (t.g.Ng[2] - 2);
+        return t;
+    }
+    static dim3 cnn_blocks(const CnnTiles &t) { return dim3((unsigned)std::min<long>(2048, (t.ntile + 3) / 4)); }
+    static int cnn_alloc(ia_ctx *c) {
+        auto &n = c->cnn;
+        size_t ns = c->g.ns;
+        for (auto &p : n.act) {
+            HIPCHK(c, hipMalloc(&p, ns * CNN_C * 2));
+            HIPCHK(c, hipMemsetAsync(p, 0, ns * CNN_C * 2, c->s));
+        }
+        for (auto &p : n.grad) {
+            HIPCHK(c, hipMalloc(&p, ns * CNN_C * 2));
+            HIPCHK(c, hipMemsetAsync(p, 0, ns * CNN_C * 2, c->s));
+        }
+        HIPCHK(c, hipMalloc((void **)&n.xkeep, ns * sizeof(double)));
+        HIPCHK(c, hipMalloc((void **)&n.gtmp, ns * sizeof(float)));
+        HIPCHK(c, hipMemsetAsync(n.gtmp, 0, ns * sizeof(float), c->s));
+        HIPCHK(c, hipMalloc((void **)&n.io, 3 * ns * sizeof(float)));
+        float **par[] = {&n.w, &n.m, &n.v, &n.gw};
+        for (float **p : par) {
+            HIPCHK(c, hipMalloc((void **)p, CNN_NPAR * sizeof(float)));
+            HIPCHK(c, hipMemsetAsync(*p, 0, CNN_NPAR * sizeof(float), c->s));
+        }
+        HIPCHK(c, hipMalloc((void **)&n.zbias, CNN_C * sizeof(float)));
+        HIPCHK(c, hipMemsetAsync(n.zbias, 0, CNN_C * sizeof(float), c->s));
+        HIPCHK(c, hipMalloc(&n.ff, CNN_L * CNN_T * 64 * sizeof(bf16x4)));
+        HIPCHK(c, hipMalloc(&n.fb, CNN_L * CNN_T * 64 * sizeof(bf16x4)));
+        CnnTiles t = cnn_tiles(c);
+        n.wnblk = std::min<long>(1024, t.ntile);
+        HIPCHK(c, hipMalloc((void **)&n.wpart, n.wnblk * (CNN_W + CNN_C) * sizeof(float)));
+        return IA_OK;
+    }
+    static void cnn_frags(ia_ctx *c) {
+        auto &n = c->cnn;
+        for (int l = 0; l < CNN_L; ++l)
+            hipLaunchKernelGGL(k_cnn_frag, dim3(CNN_T), dim3(64), 0, c->s, n.w + (long)l * CNN_W,
+                               (bf16x4 *)n.ff + l * CNN_T * 64, (bf16x4 *)n.fb + l * CNN_T * 64);
+    }
+    static void cnn_forward(ia_ctx *c) {
+        auto &n = c->cnn;
+        CnnTiles t = cnn_tiles(c);
+        for (int l = 0; l < CNN_L; ++l)
+            hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)n.act[l],
+                               (unsigned short *)n.act[l + 1], (const bf16x4 *)n.ff + l * CNN_T * 64,
+                               n.w + (long)CNN_L * CNN_W + l * CNN_C, l < CNN_L - 1 ? 1 : 0);
+    }
+    // gradients of every parameter into gw, from the output gradient in grad[0]
+    static void cnn_backward(ia_ctx *c) {
+        auto &n = c->cnn;
+        CnnTiles t = cnn_tiles(c);
+        int cur = 0;
+        for (int l = CNN_L - 1; l >= 0; --l) {
+            hipLaunchKernelGGL(k_cnn_wgrad, dim3((unsigned)n.wnblk), dim3(256), 0, c->s, t,
+                               (const unsigned short *)n.act[l], (const unsigned short *)n.grad[cur], n.wpart);
+            hipLaunchKernelGGL(k_cnn_wgrad_sum, dim3((CNN_W + CNN_C + 255) / 256), dim3(256), 0, c->s, n.wpart, n.wnblk,
+                               n.gw + (long)l * CNN_W, n.gw + (long)CNN_L * CNN_W + l * CNN_C);
+            if (l == 0) break;
+            hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)n.grad[cur],
+                               (unsigned short *)n.grad[1 - cur], (const bf16x4 *)n.fb + l * CNN_T * 64, n.zbias, 0);
+            hipLaunchKernelGGL(k_cnn_relu_bwd, dim3((unsigned)((t.ntile * 64 + 255) / 256)), dim3(256), 0, c->s, t,
+                               (unsigned short *)n.grad[1 - cur], (const unsigned short *)n.act[l]);
+            cur ^= 1;
+        }
+    }
+    static void cnn_adam(ia_ctx *c) {
+        auto &n = c->cnn;
+        n.step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        float c1 = (float)(1 - pow(b1, (double)n.step)), c2 = (float)(1 - pow(b2, (double)n.step));
+        hipLaunchKernelGGL(k_cnn_adam, dim3((CNN_NPAR + 255) / 256), dim3(256), 0, c->s, n.w, n.gw, n.m, n.v, CNN_NPAR,
+                           (float)n.lr, b1, b2, c1, c2, 1e-8f);
+        cnn_frags(c);
+    }
+    // the learned warm start: propose x += s_r·net(f, z, r), keep it if the
+    // FP64 residual dropped; optionally one training step on this solve
+    static int cnn_prelude(ia_ctx *c, double *rr) {
+        auto &n = c->cnn;
+        if (!n.ready) return fail(c, IA_ERR_BADARG, "learned Poisson mode needs ia_learned_init first");
+        Level &l0 = c->lev[0];
+        dim3 B = cb(l0);
+        CnnSolve<T> a;
+        a.f = fview(c);
+        a.t = cnn_tiles(c);
+        a.fv = P<T>(c->fb[c->fi]);
+        double rr0 = *rr, rr1 = 0;
+        hipLaunchKernelGGL(k_cnn_features<T>, B, dim3(NT), 0, c->s, a, (unsigned short *)n.act[0], pp(c, PP_FZ),
+                           pp(c, PP_FRR));
+        cnn_forward(c);
+        hipLaunchKernelGGL(k_cnn_apply<T>, B, dim3(NT), 0, c->s, a, (const unsigned short *)n.act[CNN_L], n.xkeep);
+        if (int e = true_resid(c, &rr1)) return e;
+        if (n.train) {
+            hipLaunchKernelGGL(k_cnn_loss_grad<T>, B, dim3(NT), 0, c->s, a, n.gtmp, rr0);
+            hipLaunchKernelGGL(k_cnn_loss_grad2<T>, B, dim3(NT), 0, c->s, a, (unsigned short *)n.grad[0], n.gtmp);
+            cnn_backward(c);
+            cnn_adam(c);
+        }
+        n.projections++;
+        n.last_ratio = rr0 > 0 ? rr1 / rr0 : 0;
+        if (rr1 < rr0) {
+            n.accepted++;
+            *rr = rr1;
+        } else {
+            hipLaunchKernelGGL(k_cnn_restore<T>, B, dim3(NT), 0, c->s, a, n.xkeep);
+            if (int e = true_resid(c, rr)) return e;
+        }
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
+    static int cnn_init(ia_ctx *c, unsigned seed) {
+        auto &n = c->cnn;
+        if (!n.act[0])
+            if (int e = cnn_alloc(c)) return e;
+        std::vector<float> w(CNN_NPAR, 0.f);
+        uint64_t s = 0x9E3779B97F4A7C15ull ^ seed;
+        auto uni = [&]() {  // splitmix64 -> (0,1)
+            s += 0x9E3779B97F4A7C15ull;
+            uint64_t z = s;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        };
+        auto gauss = [&]() { return sqrt(-2 * log(uni())) * cos(2 * M_PI * uni()); };
+        for (int l = 0; l < CNN_L; ++l) {
+            int cin = l == 0 ? CNN_IN : CNN_C, cout = l == CNN_L - 1 ? CNN_OUT : CNN_C;
+            double sd = sqrt(2.0 / (CNN_T * cin)) * (l == CNN_L - 1 ? 0.1 : 1.0);
+            for (int t = 0; t < CNN_T; ++t)
+                for (int ci = 0; ci < cin; ++ci)
+                    for (int co = 0; co < cout; ++co) w[(long)l * CNN_W + (t * CNN_C + ci) * CNN_C + co] = (float)(sd * gauss());
+        }
+        return cnn_set(c, w.data());
+    }
+    static int cnn_set(ia_ctx *c, const float *w) {
+        auto &n = c->cnn;
+        if (!n.act[0])
+            if (int e = cnn_alloc(c)) return e;
+        HIPCHK(c, hipMemcpyAsync(n.w, w, CNN_NPAR * sizeof(float), hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, hipMemsetAsync(n.m, 0, CNN_NPAR * sizeof(float), c->s));
+        HIPCHK(c, hipMemsetAsync(n.v, 0, CNN_NPAR * sizeof(float), c->s));
+        n.step = 0;
+        cnn_frags(c);
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        n.ready = true;
+        return IA_OK;
+    }
+    static int cnn_pack(ia_ctx *c, const float *host, int nch, void *X) {
+        long n = interior(c->g);
+        HIPCHK(c, hipMemcpyAsync(c->cnn.io, host, nch * n * sizeof(float), hipMemcpyHostToDevice, c->s));
+        hipLaunchKernelGGL(k_cnn_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->s, c->g, c->cnn.io, nch,
+                           (unsigned short *)X);
+        return IA_OK;
+    }
+    static int cnn_test_forward(ia_ctx *c, const float *feat, float *out) {
+        auto &n = c->cnn;
+        if (!n.ready) return fail(c, IA_ERR_BADARG, "ia_learned_init first");
+        long ncell = interior(c->g);
+        if (int e = cnn_pack(c, feat, CNN_IN, n.act[0])) return e;
+        cnn_forward(c);
+        hipLaunchKernelGGL(k_cnn_unpack, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->s, c->g,
+                           (const unsigned short *)n.act[CNN_L], CNN_OUT, n.io);
+        HIPCHK(c, hipMemcpyAsync(out, n.io, CNN_OUT * ncell * sizeof(float), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return IA_OK;
+    }
+    static int cnn_test_grad(ia_ctx *c, const float *feat, const float *dout, float *gw) {
+        auto &n = c->cnn;
+        if (!n.ready) return fail(c, IA_ERR_BADARG, "ia_learned_init first");
+        if (int e = cnn_pack(c, feat, CNN_IN, n.act[0])) return e;
+        cnn_forward(c);
+        if (int e = cnn_pack(c, dout, CNN_OUT, n.grad[0])) return e;
+        cnn_backward(c);
+        HIPCHK(c, hipMemcpyAsync(gw, n.gw, CNN_NPAR * sizeof(float), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
         return IA_OK;
     }
     static int project(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
@@ -1016,7 +1215,9 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     if (p->normal_scheme < 0 || p->normal_scheme > 1 || p->limiter < 0 || p->limiter > 2) return IA_ERR_BADARG;
     if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PLAIN) return IA_ERR_BADARG;
     if (p->poisson_mode != IA_POISSON_REFERENCE && p->dtype != IA_F32) return IA_ERR_BADARG;
-    if (p->poisson_mode == IA_POISSON_LEARNED) return IA_ERR_BADARG;  // needs a trained network: not available
+    if (p->poisson_mode == IA_POISSON_LEARNED &&
+        !(p->D == 3 && p->N[0] % 16 == 0 && p->N[1] % 2 == 0 && p->N[2] % 2 == 0))
+        return IA_ERR_BADARG;  // the network runs on 3D grids of 16-cell rows and even sides
     ia_ctx *c = new ia_ctx();
     c->P = *p;
     if (c->P.poisson_tol <= 0) c->P.poisson_tol = 1e-4;
@@ -1068,6 +1269,16 @@ void ia_destroy(ia_ctx *c) {
     }
     for (auto &p : c->prof)
         for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
+    {
+        auto &n = c->cnn;
+        for (void *p : n.act)
+            if (p) (void)hipFree(p);
+        for (void *p : n.grad)
+            if (p) (void)hipFree(p);
+        void *cp[] = {n.xkeep, n.gtmp, n.io, n.w, n.m, n.v, n.gw, n.zbias, n.wpart, n.ff, n.fb};
+        for (void *p : cp)
+            if (p) (void)hipFree(p);
+    }
     for (hipEvent_t e : c->it_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->pin) (void)hipHostFree(c->pin);
@@ -1157,6 +1368,50 @@ int ia_synchronize(ia_ctx *c) {
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return IA_OK;
+}
+int ia_learned_init(ia_ctx *c, unsigned seed) {
+    if (!c || c->P.dtype != IA_F32 || c->g.D != 3 || (c->g.Ng[0] - 2) % 16) return IA_ERR_BADARG;
+    (void)hipSetDevice(c->dev);
+    return Impl<float>::cnn_init(c, seed);
+}
+int ia_learned_set_weights(ia_ctx *c, const float *w, size_t n) {
+    if (!c || !w || c->P.dtype != IA_F32 || c->g.D != 3 || (c->g.Ng[0] - 2) % 16) return IA_ERR_BADARG;
+    if (n != (size_t)CNN_NPAR) return fail(c, IA_ERR_BADARG, "ia_learned_set_weights: %zu values, expected %d", n, CNN_NPAR);
+    (void)hipSetDevice(c->dev);
+    return Impl<float>::cnn_set(c, w);
+}
+int ia_learned_get_weights(ia_ctx *c, float *w, size_t n) {
+    if (!c || !w || !c->cnn.ready) return IA_ERR_BADARG;
+    if (n != (size_t)CNN_NPAR) return fail(c, IA_ERR_BADARG, "ia_learned_get_weights: %zu values, expected %d", n, CNN_NPAR);
+    (void)hipSetDevice(c->dev);
+    HIPCHK(c, hipMemcpyAsync(w, c->cnn.w, CNN_NPAR * sizeof(float), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return IA_OK;
+}
+int ia_learned_train(ia_ctx *c, int on, double lr) {
+    if (!c || !c->cnn.ready || !(lr > 0)) return IA_ERR_BADARG;
+    c->cnn.train = on != 0;
+    c->cnn.lr = lr;
+    return IA_OK;
+}
+int ia_learned_stats(ia_ctx *c, ia_learned_info *s) {
+    if (!c || !s) return IA_ERR_BADARG;
+    s->projections = c->cnn.projections;
+    s->accepted = c->cnn.accepted;
+    s->train_steps = (int)c->cnn.step;
+    s->last_ratio = c->cnn.last_ratio;
+    return IA_OK;
+}
+int ia_learned_forward(ia_ctx *c, const float *feat, float *out) {
+    if (!c || !feat || !out || c->P.dtype != IA_F32 || c->g.D != 3 || (c->g.Ng[0] - 2) % 16) return IA_ERR_BADARG;
+    (void)hipSetDevice(c->dev);
+    return Impl<float>::cnn_test_forward(c, feat, out);
+}
+int ia_learned_grad(ia_ctx *c, const float *feat, const float *dout, float *gw) {
+    if (!c || !feat || !dout || !gw || c->P.dtype != IA_F32 || c->g.D != 3 || (c->g.Ng[0] - 2) % 16)
+        return IA_ERR_BADARG;
+    (void)hipSetDevice(c->dev);
+    return Impl<float>::cnn_test_grad(c, feat, dout, gw);
 }
 int ia_profile_enable(ia_ctx *c, int kernel, int on) {
     if (!c || kernel < 0 || kernel >= IA_PROF_NCLASS) return IA_ERR_BADARG;

@@ -1,0 +1,300 @@
+// ia_learned.h -- the learned Poisson correction: a five-layer 3x3x3
+// convolutional network (3 -> 16 -> 16 -> 16 -> 16 -> 2 channels) evaluated
+// with BF16 matrix-core (MFMA) products and FP32 accumulation, trained on
+// the device from the solver's own residual.  It has no counterpart in the
+// reference: it proposes a correction to the pressure warm start, which is
+// kept only when it lowers the FP64 residual; the flexible CG then solves to
+// the requested tolerance either way.
+//
+// Tensors are channels-last BF16 over the ghost-inclusive level-0 grid,
+// 16 channels per voxel (unused channels hold zero); ghost voxels are never
+// written and stay zero, which is the zero padding of the convolution.
+// Weights are W[layer][tap][ci][co] in FP32 (tap = (dz+1)*9 + (dy+1)*3 + dx+1),
+// mirrored into BF16 MFMA operand order for the forward and the transposed
+// convolutions.
+#pragma once
+#include "ia_poisson.h"
+
+namespace ia {
+
+constexpr int CNN_C = 16;                         // channels of every tensor
+constexpr int CNN_L = 5;                          // layers
+constexpr int CNN_T = 27;                         // taps
+constexpr int CNN_W = CNN_T * CNN_C * CNN_C;      // weights per layer
+constexpr int CNN_NPAR = CNN_L * (CNN_W + CNN_C); // weights + biases
+constexpr int CNN_IN = 3, CNN_OUT = 2;            // real input / output channels
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short f2bf(float f) {  // round to nearest even
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// one 16-voxel row segment (x0 .. x0+15 at y, z) per tile
+struct CnnTiles {
+    Grid g;
+    int tx;      // tiles per row = N0 / 16
+    long ntile;  // tx * N1 * N2
+};
+__device__ inline long tile_base(const CnnTiles &t, long tile) {
+    long r = tile / t.tx;
+    int xt = (int)(tile % t.tx);
+    int y = 1 + (int)(r % (t.g.Ng[1] - 2));
+    int z = 1 + (int)(r / (t.g.Ng[1] - 2));
+    return (long)(1 + 16 * xt) + t.g.st[1] * y + t.g.st[2] * z;
+}
+__device__ inline long tap_off(const Grid &g, int t) {
+    int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+    return (long)dx + g.st[1] * dy + g.st[2] * dz;
+}
+
+// Y = act(conv(X, W) + b) on interior voxels.  Per wave and tile, the 27
+// taps are 27 MFMA 16x16x16 products D[co][vox] += A[co][ci] B[ci][vox] with
+// A the tap's weights (lane l: co = l%16, ci = 4(l/16)..+3) and B the input
+// rows (lane l: voxel l%16, channels 4(l/16)..+3, one 8-byte load); lane l
+// then holds channels 4(l/16)..+3 of voxel l%16 and stores them as 8 bytes.
+__global__ void __launch_bounds__(256) k_cnn_conv(CnnTiles t, const unsigned short *__restrict__ X,
+                                                  unsigned short *__restrict__ Y,
+                                                  const bf16x4 *__restrict__ Wf,  // [27][64]
+                                                  const float *__restrict__ bias, int relu) {
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long nwave = ((long)gridDim.x * blockDim.x) >> 6;
+    const int j = lane & 15, kg = lane >> 4;
+    bf16x4 a[CNN_T];
+#pragma unroll
+    for (int q = 0; q < CNN_T; ++q) a[q] = Wf[q * 64 + lane];
+    float b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = bias[4 * kg + q];
+    long off[CNN_T];
+#pragma unroll
+    for (int q = 0; q < CNN_T; ++q) off[q] = tap_off(t.g, q);
+    for (long tile = wave; tile < t.ntile; tile += nwave) {
+        long V = tile_base(t, tile) + j;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < CNN_T; ++q) {
+            bf16x4 bx = *(const bf16x4 *)(X + (V + off[q]) * CNN_C + 4 * kg);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bx, acc, 0, 0, 0);
+        }
+        unsigned short o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v = acc[q] + b[q];
+            if (relu && v < 0.f) v = 0.f;
+            o[q] = f2bf(v);
+        }
+        *(uint2 *)(Y + V * CNN_C + 4 * kg) = make_uint2(o[0] | ((unsigned)o[1] << 16), o[2] | ((unsigned)o[3] << 16));
+    }
+}
+
+// MFMA operand order of one layer's weights: forward A[co][ci] per tap, and
+// the transposed convolution's A[ci][co] of the flipped tap
+__global__ void k_cnn_frag(const float *__restrict__ W, bf16x4 *__restrict__ Ff, bf16x4 *__restrict__ Fb) {
+    int t = blockIdx.x, lane = threadIdx.x;  // 27 blocks of 64
+    int j = lane & 15, kg = lane >> 4;
+    unsigned short f[4], r[4];
+    for (int q = 0; q < 4; ++q) {
+        f[q] = f2bf(W[(t * CNN_C + 4 * kg + q) * CNN_C + j]);                 // W[t][ci=4kg+q][co=j]
+        r[q] = f2bf(W[((CNN_T - 1 - t) * CNN_C + j) * CNN_C + 4 * kg + q]);   // W[26-t][ci=j][co=4kg+q]
+    }
+    Ff[t * 64 + lane] = bf16x4{(short)f[0], (short)f[1], (short)f[2], (short)f[3]};
+    Fb[t * 64 + lane] = bf16x4{(short)r[0], (short)r[1], (short)r[2], (short)r[3]};
+}
+
+// dH = dY ⊙ (H > 0): the ReLU derivative, read from the layer's output
+__global__ void __launch_bounds__(256) k_cnn_relu_bwd(CnnTiles t, unsigned short *__restrict__ dY,
+                                                      const unsigned short *__restrict__ H) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (tile, lane): 16 voxels x 16 ch
+    if (i >= t.ntile * 64) return;
+    long tile = i >> 6;
+    int lane = (int)(i & 63), j = lane & 15, kg = lane >> 4;
+    long o = (tile_base(t, tile) + j) * CNN_C + 4 * kg;
+    for (int q = 0; q < 4; ++q)
+        if (!(bf2f(H[o + q]) > 0.f)) dY[o + q] = 0;
+}
+
+// weight and bias gradient partials of one layer:
+// gW[t][ci][co] = Σ_v X[v+off_t][ci] dY[v][co], gb[co] = Σ_v dY[v][co].
+// Thread (ci, co) of a block sums over the block's tiles with the tile's
+// inputs and output gradients staged in LDS; partials per block.
+__global__ void __launch_bounds__(256) k_cnn_wgrad(CnnTiles t, const unsigned short *__restrict__ X,
+                                                   const unsigned short *__restrict__ dY,
+                                                   float *__restrict__ part /* [nblk][27*256 + 16] */) {
+    __shared__ float xs[3][3][18][CNN_C];  // input rows y-1..y+1, z-1..z+1, x0-1..x0+16
+    __shared__ float ds[16][CNN_C];
+    const int ci = threadIdx.x >> 4, co = threadIdx.x & 15;
+    float acc[CNN_T];
+#pragma unroll
+    for (int q = 0; q < CNN_T; ++q) acc[q] = 0.f;
+    float accb = 0.f;
+    const Grid &g = t.g;
+    for (long tile = blockIdx.x; tile < t.ntile; tile += gridDim.x) {
+        long B0 = tile_base(t, tile);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 9 * 18 * CNN_C; e += 256) {
+            int c = e % CNN_C, xx = (e / CNN_C) % 18, yz = e / (CNN_C * 18);
+            int yy = yz % 3, zz = yz / 3;
+            long V = B0 + (xx - 1) + g.st[1] * (yy - 1) + g.st[2] * (zz - 1);
+            xs[zz][yy][xx][c] = bf2f(X[V * CNN_C + c]);
+        }
+        for (int e = threadIdx.x; e < 16 * CNN_C; e += 256) ds[e / CNN_C][e % CNN_C] = bf2f(dY[(B0 + e / CNN_C) * CNN_C + e % CNN_C]);
+        __syncthreads();
+        for (int v = 0; v < 16; ++v) {
+            float d = ds[v][co];
+            accb += d;
+#pragma unroll
+            for (int q = 0; q < CNN_T; ++q) acc[q] += xs[q / 9][(q / 3) % 3][v + q % 3][ci] * d;
+        }
+    }
+    float *P = part + (long)blockIdx.x * (CNN_W + CNN_C);
+#pragma unroll
+    for (int q = 0; q < CNN_T; ++q) P[(q * CNN_C + ci) * CNN_C + co] = acc[q];
+    if (ci == 0) P[CNN_W + co] = accb;  // thread (0, co) saw every voxel's dY[co] (ci plays no part)
+}
+// fixed-order sum of the per-block partials
+__global__ void __launch_bounds__(256) k_cnn_wgrad_sum(const float *__restrict__ part, long nblk,
+                                                       float *__restrict__ gW, float *__restrict__ gb) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= CNN_W + CNN_C) return;
+    double s = 0;
+    for (long b = 0; b < nblk; ++b) s += part[b * (CNN_W + CNN_C) + i];
+    if (i < CNN_W) gW[i] = (float)s;
+    else gb[i - CNN_W] = (float)s;
+}
+
+// Adam on every parameter; frozen entries (inputs that are always zero,
+// outputs that are never used) keep a zero gradient and stay zero
+__global__ void k_cnn_adam(float *__restrict__ w, const float *__restrict__ gw, float *__restrict__ m,
+                           float *__restrict__ v, int n, float lr, float b1, float b2, float c1, float c2,
+                           float eps) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = gw[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// solver side: features, applying the correction, the loss gradient
+// ---------------------------------------------------------------------------
+// RMS of the interior of a FP64 field, from partials of Σa²
+template <class T> struct CnnSolve {
+    FcgV<T> f;
+    CnnTiles t;
+    const T *fv;  // volume fraction
+};
+// features: ch0 = f, ch1 = z/s_z, ch2 = r/s_r (r = z - A x0), other channels 0
+template <class T>
+__global__ void __launch_bounds__(256) k_cnn_features(CnnSolve<T> a, unsigned short *__restrict__ X0,
+                                                      const double *part_zz, const double *part_rr) {
+    double sz = sqrt(block_total(part_zz, a.f.t.nblk) / (double)a.f.ncell);
+    double sr = sqrt(block_total(part_rr, a.f.t.nblk) / (double)a.f.ncell);
+    sz = sz > 0 ? sz : 1.0;
+    sr = sr > 0 ? sr : 1.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.f.S->sload = sr;
+    COL_LOOP(a.f, {
+        unsigned short o[CNN_C];
+        for (int c = 0; c < CNN_C; ++c) o[c] = 0;
+        o[0] = f2bf((float)a.fv[I]);
+        o[1] = f2bf((float)(a.f.z[I] / sz));
+        o[2] = f2bf((float)(a.f.r[I] / sr));
+        uint4 *d = (uint4 *)(X0 + I * CNN_C);
+        d[0] = make_uint4(o[0] | ((unsigned)o[1] << 16), o[2] | ((unsigned)o[3] << 16), o[4] | ((unsigned)o[5] << 16),
+                          o[6] | ((unsigned)o[7] << 16));
+        d[1] = make_uint4(o[8] | ((unsigned)o[9] << 16), o[10] | ((unsigned)o[11] << 16),
+                          o[12] | ((unsigned)o[13] << 16), o[14] | ((unsigned)o[15] << 16));
+    })
+}
+// the 2x2x2 block average of channel 1 at voxel I (the coarse correction);
+// block-averaging is symmetric, so the same operator serves the backward pass
+__device__ inline float blockavg(const Grid &g, const unsigned short *Y, int ch, const int p[3]) {
+    int b0 = ((p[0] - 1) & ~1) + 1, b1 = ((p[1] - 1) & ~1) + 1, b2 = ((p[2] - 1) & ~1) + 1;
+    float s = 0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+                s += bf2f(Y[((long)(b0 + dx) + g.st[1] * (b1 + dy) + g.st[2] * (b2 + dz)) * CNN_C + ch]);
+    return s * 0.125f;
+}
+// x_keep = x; x += iD (2/3 r + s_r (out0 + avg(out1))): the proposed warm
+// start is a damped Jacobi step on the residual r = z - A x plus the
+// network's correction of it, in the same Jacobi-scaled units
+template <class T>
+__global__ void __launch_bounds__(256) k_cnn_apply(CnnSolve<T> a, const unsigned short *__restrict__ Y,
+                                                   double *__restrict__ xkeep) {
+    double sr = a.f.S->sload;
+    COL_LOOP(a.f, {
+        float d = bf2f(Y[I * CNN_C]) + blockavg(a.f.g, Y, 1, p);
+        double id = (double)a.f.iD[I];
+        xkeep[I] = a.f.x[I];
+        a.f.x[I] += id * ((2.0 / 3.0) * a.f.r[I] + sr * (double)d);
+    })
+}
+// restore the warm start (the proposal did not lower the residual)
+template <class T>
+__global__ void __launch_bounds__(256) k_cnn_restore(CnnSolve<T> a, const double *__restrict__ xkeep) {
+    COL_LOOP(a.f, { a.f.x[I] = xkeep[I]; })
+}
+// loss ‖r1‖²/‖r0‖² with r1 = z - A x'; d loss/d x' = -2 A r1 / ‖r0‖² (A is
+// symmetric); the output gradient is that times s_r iD (channel 0) and the
+// block average of it (channel 1)
+template <class T>
+__global__ void __launch_bounds__(256) k_cnn_loss_grad(CnnSolve<T> a, float *__restrict__ gtmp, double r0) {
+    double sc = -2.0 * a.f.S->sload / (r0 > 0 ? r0 : 1.0);
+    const Grid &g = a.f.g;
+    COL_LOOP(a.f, {
+        double Ar = g.D == 3 ? mult64_d<3>(g, p, I, a.f.L, a.f.r) : mult64_d<2>(g, p, I, a.f.L, a.f.r);
+        gtmp[I] = (float)(sc * (double)a.f.iD[I] * Ar);
+    })
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_cnn_loss_grad2(CnnSolve<T> a, unsigned short *__restrict__ dY,
+                                                        const float *__restrict__ gtmp) {
+    const Grid &g = a.f.g;
+    COL_LOOP(a.f, {
+        int b0 = ((p[0] - 1) & ~1) + 1, b1 = ((p[1] - 1) & ~1) + 1, b2 = ((p[2] - 1) & ~1) + 1;
+        float s = 0;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) s += gtmp[(long)(b0 + dx) + g.st[1] * (b1 + dy) + g.st[2] * (b2 + dz)];
+        unsigned short o[CNN_C];
+        for (int c = 0; c < CNN_C; ++c) o[c] = 0;
+        o[0] = f2bf(gtmp[I]);
+        o[1] = f2bf(s * 0.125f);
+        uint4 *d = (uint4 *)(dY + I * CNN_C);
+        d[0] = make_uint4(o[0] | ((unsigned)o[1] << 16), 0u, 0u, 0u);
+        d[1] = make_uint4(0u, 0u, 0u, 0u);
+    })
+}
+
+// test entry points: interior planes (x fastest, channel planes last) <-> tensor
+__global__ void __launch_bounds__(256) k_cnn_pack(Grid g, const float *__restrict__ src, int nch,
+                                                  unsigned short *__restrict__ X) {
+    long n0 = g.Ng[0] - 2, n1 = g.Ng[1] - 2, n2 = g.Ng[2] - 2, n = n0 * n1 * n2;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+    long V = (x + 1) + g.st[1] * (y + 1) + g.st[2] * (z + 1);
+    for (int c = 0; c < CNN_C; ++c) X[V * CNN_C + c] = c < nch ? f2bf(src[c * n + i]) : (unsigned short)0;
+}
+__global__ void __launch_bounds__(256) k_cnn_unpack(Grid g, const unsigned short *__restrict__ X, int nch,
+                                                    float *__restrict__ dst) {
+    long n0 = g.Ng[0] - 2, n1 = g.Ng[1] - 2, n2 = g.Ng[2] - 2, n = n0 * n1 * n2;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long x = i % n0, y = (i / n0) % n1, z = i / (n0 * n1);
+    long V = (x + 1) + g.st[1] * (y + 1) + g.st[2] * (z + 1);
+    for (int c = 0; c < nch; ++c) dst[c * n + i] = bf2f(X[V * CNN_C + c]);
+}
+
+}  // namespace ia
