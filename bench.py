@@ -169,6 +169,42 @@ def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
                        f"{t:.1f} s, CG iterations per solve {its}")
 
 
+def poisson_compare(ia, N, rtol, warm=4, steps=3, lr=1e-3):
+    """Time-to-rtol of the pressure solve for the plain multigrid, the flexible
+    CG and the learned-correction modes on the same case and the same step
+    range (the learned network trains during the warm-up steps, then is frozen)."""
+    out = {}
+    for mode in ("plain", "refined", "learned"):
+        sim, _, _ = build_sim(ia, N, mode, rtol)
+        if mode == "learned":
+            sim.learned_init(seed=0)
+            sim.learned_train(True, lr=lr)
+        for _ in range(warm):
+            ia.sim_step_(sim)
+        if mode == "learned":
+            sim.learned_train(False)
+            trained = sim.learned_stats()
+        ms, its, rel = [], [], []
+        for _ in range(steps):
+            st = ia.MPFMomStep_(sim, sim, sim)
+            for q in range(2):
+                ms.append(st.solve[q].ms)
+                its.append(st.solve[q].iters)
+                rel.append(st.solve[q].rel)
+        rec = dict(ms=round(float(np.mean(ms)), 3), iterations=float(np.mean(its)), rel_max=float(np.max(rel)))
+        if mode == "learned":
+            s = sim.learned_stats()
+            rec.update(train_steps=s["train_steps"], accepted=s["accepted"] - trained["accepted"],
+                       timed_solves=len(ms), last_residual_ratio=round(s["last_ratio"], 4))
+        out[mode] = rec
+        sim.close()
+        print(f"[bench] poisson {mode}: {rec}", file=sys.stderr, flush=True)
+    out["learned_vs_plain_speedup"] = round(out["plain"]["ms"] / out["learned"]["ms"], 3)
+    out["refined_vs_plain_speedup"] = round(out["plain"]["ms"] / out["refined"]["ms"], 3)
+    out["steps"] = f"{warm} warm-up (learned: training) + {steps} timed steps, mean over the timed solves"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +218,7 @@ def main():
     ap.add_argument("--cpu-N", type=int, default=192)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain / refined / learned solver comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the path shards by z-slabs across GPUs by design, "
@@ -250,6 +287,8 @@ def main():
         "kernels": timed,
     }
     sim.close()
+    if not args.no_compare:
+        out["config"]["poisson_compare"] = poisson_compare(ia, N, args.rtol)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps)
     print(json.dumps(out), flush=True)
