@@ -169,7 +169,7 @@ def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
                        f"{t:.1f} s, CG iterations per solve {its}")
 
 
-def poisson_compare(ia, N, rtol, warm=4, steps=3, lr=1e-3):
+def poisson_compare(ia, N, rtol, warm=8, steps=3, lr=1e-3):
     """Time-to-rtol of the pressure solve for the plain multigrid, the flexible
     CG and the learned-correction modes on the same case and the same step
     range (the learned network trains during the warm-up steps, then is frozen)."""

@@ -754,13 +754,32 @@ template <class T> struct Impl {
             hipLaunchKernelGGL(k_cnn_frag, dim3(CNN_T), dim3(64), 0, c->s, n.w + (long)l * CNN_W,
                                (bf16x4 *)n.ff + l * CNN_T * 64, (bf16x4 *)n.fb + l * CNN_T * 64);
     }
+    static CnnBlocks cnn_lds_blocks(ia_ctx *c) {
+        CnnBlocks t;
+        t.g = c->lev[0].g;
+        t.tx = (t.g.Ng[0] - 2) / 16;
+        t.ty = (t.g.Ng[1] - 2) / 4;
+        t.tz = (t.g.Ng[2] - 2 + CNN_ZB - 1) / CNN_ZB;
+        return t;
+    }
+    // Y = act(conv(X) + b): the LDS-staged kernel where rows of 4 fit, else the direct one
+    static void cnn_conv(ia_ctx *c, const void *X, void *Y, const void *frag, const float *bias, int relu) {
+        Grid &g = c->lev[0].g;
+        if ((g.Ng[1] - 2) % 4 == 0) {
+            CnnBlocks t = cnn_lds_blocks(c);
+            hipLaunchKernelGGL(k_cnn_conv_lds, dim3((unsigned)((long)t.tx * t.ty * t.tz)), dim3(256), 0, c->s, t,
+                               (const unsigned short *)X, (unsigned short *)Y, (const bf16x4 *)frag, bias, relu);
+        } else {
+            CnnTiles t = cnn_tiles(c);
+            hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)X,
+                               (unsigned short *)Y, (const bf16x4 *)frag, bias, relu);
+        }
+    }
     static void cnn_forward(ia_ctx *c) {
         auto &n = c->cnn;
-        CnnTiles t = cnn_tiles(c);
         for (int l = 0; l < CNN_L; ++l)
-            hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)n.act[l],
-                               (unsigned short *)n.act[l + 1], (const bf16x4 *)n.ff + l * CNN_T * 64,
-                               n.w + (long)CNN_L * CNN_W + l * CNN_C, l < CNN_L - 1 ? 1 : 0);
+            cnn_conv(c, n.act[l], n.act[l + 1], (const bf16x4 *)n.ff + l * CNN_T * 64,
+                     n.w + (long)CNN_L * CNN_W + l * CNN_C, l < CNN_L - 1 ? 1 : 0);
     }
     // gradients of every parameter into gw, from the output gradient in grad[0]
     static void cnn_backward(ia_ctx *c) {
@@ -773,8 +792,7 @@ template <class T> struct Impl {
             hipLaunchKernelGGL(k_cnn_wgrad_sum, dim3((CNN_W + CNN_C + 255) / 256), dim3(256), 0, c->s, n.wpart, n.wnblk,
                                n.gw + (long)l * CNN_W, n.gw + (long)CNN_L * CNN_W + l * CNN_C);
             if (l == 0) break;
-            hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)n.grad[cur],
-                               (unsigned short *)n.grad[1 - cur], (const bf16x4 *)n.fb + l * CNN_T * 64, n.zbias, 0);
+            cnn_conv(c, n.grad[cur], n.grad[1 - cur], (const bf16x4 *)n.fb + l * CNN_T * 64, n.zbias, 0);
             hipLaunchKernelGGL(k_cnn_relu_bwd, dim3((unsigned)((t.ntile * 64 + 255) / 256)), dim3(256), 0, c->s, t,
                                (unsigned short *)n.grad[1 - cur], (const unsigned short *)n.act[l]);
             cur ^= 1;

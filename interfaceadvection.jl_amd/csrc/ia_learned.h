@@ -94,6 +94,75 @@ __global__ void __launch_bounds__(256) k_cnn_conv(CnnTiles t, const unsigned sho
     }
 }
 
+// The same convolution with the input staged through LDS: a block owns a
+// 16 (x) by 4 (y) column of voxels and walks CNN_ZB planes in z.  A ring of
+// four LDS planes holds the input rows y-1..y+4, x-1..x+16 of planes z-1,
+// z, z+1 and the prefetched z+2; every tap's MFMA operand is one 8-byte LDS
+// read, and each input plane is read from memory once per block.
+constexpr int CNN_ZB = 32;
+struct CnnBlocks {
+    Grid g;
+    int tx, ty, tz;  // x tiles of 16, y tiles of 4, z tiles of CNN_ZB
+};
+__global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigned short *__restrict__ X,
+                                                      unsigned short *__restrict__ Y,
+                                                      const bf16x4 *__restrict__ Wf, const float *__restrict__ bias,
+                                                      int relu) {
+    constexpr int PX = 18, PY = 6, PV = PX * PY;  // voxels per staged plane
+    __shared__ __align__(16) unsigned short pl[4][PV * CNN_C];
+    const Grid &g = t.g;
+    long b = blockIdx.x;
+    int bx = (int)(b % t.tx), by = (int)((b / t.tx) % t.ty), bz = (int)(b / ((long)t.tx * t.ty));
+    int x0 = 1 + 16 * bx, y0 = 1 + 4 * by, zlo = 1 + CNN_ZB * bz;
+    int zhi = min(zlo + CNN_ZB - 1, g.Ng[2] - 2);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = lane & 15, kg = lane >> 4;
+    // staging: thread s < 216 moves 16 bytes (half a voxel's channels) of a plane
+    const int s = threadIdx.x;
+    const bool stg = s < 2 * PV;
+    const int sv = s >> 1, sh = s & 1;
+    const long sI = stg ? (long)(x0 - 1 + sv % PX) + g.st[1] * (y0 - 1 + sv / PX) : 0;
+    auto fetch = [&](int z) -> uint4 {
+        if (!stg) return make_uint4(0u, 0u, 0u, 0u);
+        return *(const uint4 *)(X + (sI + g.st[2] * z) * CNN_C + 8 * sh);
+    };
+    auto put = [&](int slot, uint4 v) {
+        if (stg) *(uint4 *)(&pl[slot][sv * CNN_C + 8 * sh]) = v;
+    };
+    bf16x4 a[CNN_T];
+#pragma unroll
+    for (int q = 0; q < CNN_T; ++q) a[q] = Wf[q * 64 + lane];
+    float bb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bb[q] = bias[4 * kg + q];
+    put((zlo - 1) & 3, fetch(zlo - 1));
+    put(zlo & 3, fetch(zlo));
+    put((zlo + 1) & 3, fetch(zlo + 1));
+    for (int z = zlo; z <= zhi; ++z) {
+        __syncthreads();
+        uint4 nxt = make_uint4(0u, 0u, 0u, 0u);
+        if (z + 2 <= zhi + 1) nxt = fetch(z + 2);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < CNN_T; ++q) {
+            int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+            const unsigned short *src = &pl[(z + dz) & 3][((w + 1 + dy) * PX + (j + 1 + dx)) * CNN_C + 4 * kg];
+            bf16x4 bx = *(const bf16x4 *)src;
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bx, acc, 0, 0, 0);
+        }
+        unsigned short o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v = acc[q] + bb[q];
+            if (relu && v < 0.f) v = 0.f;
+            o[q] = f2bf(v);
+        }
+        long V = (long)(x0 + j) + g.st[1] * (y0 + w) + g.st[2] * z;
+        *(uint2 *)(Y + V * CNN_C + 4 * kg) = make_uint2(o[0] | ((unsigned)o[1] << 16), o[2] | ((unsigned)o[3] << 16));
+        if (z + 2 <= zhi + 1) put((z + 2) & 3, nxt);
+    }
+}
+
 // MFMA operand order of one layer's weights: forward A[co][ci] per tap, and
 // the transposed convolution's A[ci][co] of the flipped tap
 __global__ void k_cnn_frag(const float *__restrict__ W, bf16x4 *__restrict__ Ff, bf16x4 *__restrict__ Fb) {
