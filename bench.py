@@ -21,6 +21,7 @@ timed region), and the CPU oracle timed on the host cores over a bounded
 sample of the same workload.
 """
 import argparse
+import importlib
 import json
 import os
 import sys
@@ -32,82 +33,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 from __graft_entry__ import load_package  # noqa: E402
 
+_ia = load_package()
+pool = importlib.import_module(_ia.__name__ + ".pool")
+bubbles, pool_sdf_samples, make_case, build_sim = pool.bubbles, pool.pool_sdf_samples, pool.make_case, pool.build_sim
+
 METRIC = "Mcell-steps/s at 256^3 two-phase; Poisson time-to-1e-6 learned vs plain MG"
 HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
-
-
-def bubbles(L, nb, seed, rmin, rmax, zmax, clearance=2.0):
-    """Rejection-sampled non-overlapping bubbles inside the pool z < zmax."""
-    rng = np.random.default_rng(seed)
-    out = []
-    tries = 0
-    while len(out) < nb:
-        tries += 1
-        if tries > 200000:
-            raise RuntimeError("could not place bubbles")
-        r = rng.uniform(rmin, rmax) * L
-        c = rng.uniform(0, L, 3)
-        lo, hi = r + clearance, L - r - clearance
-        if not (lo <= c[0] <= hi and lo <= c[1] <= hi and lo <= c[2] <= zmax * L - r - clearance):
-            continue
-        if any(np.linalg.norm(c - np.array(b[:3])) < r + b[3] + clearance for b in out):
-            continue
-        out.append((c[0], c[1], c[2], r))
-    return out
-
-
-def pool_sdf_samples(N, bl, zmax, T=np.float32):
-    """applyVOF! samples (centre, ±Δx per axis) of the SDF
-    φ = max(z - zmax·L, max_b(r_b - |x - c_b|)) for every interior cell.
-
-    The SDF is evaluated exactly within 3 cells of every bubble; elsewhere the
-    plane term alone is used.  Where they differ both are below -3, and the
-    PLIC formula then gives f = 1 exactly either way, so f is identical to the
-    full evaluation."""
-    L = float(N)
-    dx = T(0.01)
-    D = 3
-    z = (np.arange(N, dtype=T) + T(0.5))
-    S = np.empty((N, N, N, 1 + 2 * D), dtype=T, order="F")
-    zc = z[None, None, :] - T(zmax * L)
-    S[..., 0] = zc
-    S[..., 1] = zc
-    S[..., 2] = zc
-    S[..., 3] = zc
-    S[..., 4] = zc
-    S[..., 5] = (z + dx).astype(T)[None, None, :] - T(zmax * L)
-    S[..., 6] = (z - dx).astype(T)[None, None, :] - T(zmax * L)
-    for (cx, cy, cz, r) in bl:
-        lo = [max(0, int(np.floor(c - r - 4))) for c in (cx, cy, cz)]
-        hi = [min(N, int(np.ceil(c + r + 4))) for c in (cx, cy, cz)]
-        ax = [np.arange(lo[a], hi[a], dtype=T) + T(0.5) for a in range(3)]
-        X = np.meshgrid(*ax, indexing="ij")
-        cc = (T(cx), T(cy), T(cz))
-        offs = [(0, 0, 0), (dx, 0, 0), (-dx, 0, 0), (0, dx, 0), (0, -dx, 0), (0, 0, dx), (0, 0, -dx)]
-        blk = S[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2], :]
-        for q, o in enumerate(offs):
-            Y = [(X[a] + T(o[a])).astype(T) for a in range(3)]
-            d = np.sqrt((Y[0] - cc[0]) ** 2 + (Y[1] - cc[1]) ** 2 + (Y[2] - cc[2]) ** 2)
-            blk[..., q] = np.maximum(blk[..., q], (T(r) - d).astype(T))
-    return np.ascontiguousarray(S.reshape(N * N * N, 1 + 2 * D, order="F"))
-
-
-def make_case(N, poisson, T=np.float32, nb=32, seed=3):
-    L = float(N)
-    bl = bubbles(N, nb, seed, 0.03, 0.08, 0.7)
-    kw = dict(T=T, λρ=1e-3, λμ=1e-2, ν=1.0 * L / 1e4, g=(0.0, 0.0, -1.0 / L), perdir=())
-    return bl, kw
-
-
-def build_sim(ia, N, poisson, rtol):
-    bl, kw = make_case(N, poisson)
-    sim = ia.TwoPhaseSimulation((N, N, N), (0.0, 0.0, 0.0), float(N), poisson=poisson, poisson_rtol=rtol,
-                                poisson_max_outer=200, **kw)
-    s = pool_sdf_samples(N, bl, 0.7)
-    sim._call("ia_apply_vof", ia._ptr(s), s.nbytes)
-    del s
-    sim.flow.Δt[-1] = min(sim.flow.Δt[-1], sim.T.type(sim.MPCFL()))
-    return sim, bl, kw
+WEIGHTS = os.path.join(ROOT, "interfaceadvection.jl_amd", "learned_pool256.f32")
 
 
 # algorithmic bytes per interior cell (FP32, 3D): each distinct array a kernel
@@ -177,7 +109,10 @@ def poisson_compare(ia, N, rtol, warm=8, steps=3, lr=1e-3):
     for mode in ("plain", "refined", "learned"):
         sim, _, _ = build_sim(ia, N, mode, rtol)
         if mode == "learned":
-            sim.learned_init(seed=0)
+            if os.path.exists(WEIGHTS):  # trained by interfaceadvection.jl_amd/train_learned.py on other layouts
+                sim.learned_weights = np.fromfile(WEIGHTS, dtype="<f4")
+            else:
+                sim.learned_init(seed=0)
             sim.learned_train(True, lr=lr)
         for _ in range(warm):
             ia.sim_step_(sim)

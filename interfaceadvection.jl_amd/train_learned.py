@@ -1,0 +1,74 @@
+"""Train the learned Poisson correction on the two-phase pool workload.
+
+The network is trained on the device, inside the solver: every projection of
+a LEARNED-mode simulation with training switched on takes one Adam step on
+the loss ‖z - A x‖² / ‖r0‖² of its own corrected warm start.  Training runs
+over several bubble layouts (seeds 1000, 1001, ...) of the 3D pool case
+(liquid below 0.7 L, 32 bubbles of radius U(0.03, 0.08) L, density ratio
+1000, viscosity ratio 100, slip walls, gravity), a number of time steps each.
+
+    python interfaceadvection.jl_amd/train_learned.py --N 256 --seeds 4 --steps 25 \\
+        --out interfaceadvection.jl_amd/learned_pool256.f32
+
+writes the parameters (IA_LEARNED_NPAR float32 values, little-endian, the
+layout of ia_learned_set_weights) and a JSON record of the run beside them.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--seeds", type=int, default=4)
+    ap.add_argument("--first-seed", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=25)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--init-seed", type=int, default=0)
+    ap.add_argument("--out", default=os.path.join(HERE, "learned_pool256.f32"))
+    args = ap.parse_args()
+    sys.path.insert(0, ROOT)
+    import importlib
+    from __graft_entry__ import load_package
+    ia = load_package()
+    pool = importlib.import_module(ia.__name__ + ".pool")
+    w = None
+    log = []
+    t0 = time.perf_counter()
+    for k in range(args.seeds):
+        seed = args.first_seed + k
+        sim, _, _ = pool.build_sim(ia, args.N, "learned", 1e-6, seed=seed)
+        if w is None:
+            sim.learned_init(seed=args.init_seed)
+        else:
+            sim.learned_weights = w
+        sim.learned_train(True, lr=args.lr)
+        ratios, iters = [], []
+        for _ in range(args.steps):
+            st = ia.MPFMomStep_(sim, sim, sim)
+            ratios.append(sim.learned_stats()["last_ratio"])
+            iters += [st.solve[0].iters, st.solve[1].iters]
+        w = sim.learned_weights
+        rec = dict(seed=seed, steps=args.steps, train_steps=sim.learned_stats()["train_steps"],
+                   residual_ratio_first=round(ratios[0], 4), residual_ratio_last=round(ratios[-1], 4),
+                   cg_iterations_mean=float(np.mean(iters)), elapsed_s=round(time.perf_counter() - t0, 1))
+        log.append(rec)
+        print("[train]", rec, flush=True)
+        sim.close()
+    w.astype("<f4").tofile(args.out)
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as fh:
+        json.dump(dict(N=args.N, lr=args.lr, init_seed=args.init_seed, runs=log,
+                       parameters=int(w.size)), fh, indent=1)
+    print(f"[train] wrote {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
