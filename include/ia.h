@@ -30,11 +30,19 @@ typedef struct ia_ctx ia_ctx;
 enum ia_dtype { IA_F32 = 0, IA_F64 = 1 };
 
 /* normalScheme plug-ins (normalEstimation.jl): WH is the reference default
- * (InterfaceAdvection.jl:70); Column is its first stage. */
-enum ia_normal_scheme { IA_NORMAL_WH = 0, IA_NORMAL_COLUMN = 1 };
+ * (InterfaceAdvection.jl:70); Column is its first stage (:75-97); WY is
+ * :35-68; MYC is Youngs + CCi (:185-257); PCD :161-165; Youngs :232-257;
+ * SLIC :172-176. */
+enum ia_normal_scheme {
+    IA_NORMAL_WH = 0, IA_NORMAL_COLUMN = 1, IA_NORMAL_WY = 2, IA_NORMAL_MYC = 3, IA_NORMAL_PCD = 4,
+    IA_NORMAL_YOUNGS = 5, IA_NORMAL_SLIC = 6
+};
 
 /* λ plug-ins (flow.jl:5-15); Koren is the reference default (InterfaceAdvection.jl:70). */
-enum ia_limiter { IA_LIMITER_KOREN = 0, IA_LIMITER_MINMOD = 1, IA_LIMITER_UPWIND = 2 };
+enum ia_limiter {
+    IA_LIMITER_KOREN = 0, IA_LIMITER_MINMOD = 1, IA_LIMITER_UPWIND = 2, IA_LIMITER_VANALBADA1 = 3,
+    IA_LIMITER_SUPERBEE = 4, IA_LIMITER_TVDCEN = 5, IA_LIMITER_TVDDOWN = 6, IA_LIMITER_SWEBY = 7
+};
 
 /* Pressure solve.
  * REFERENCE: WaterLily solver!(ml; tol, itmx) exactly as the reference calls it

@@ -126,6 +126,37 @@ def upwind(u, c, d):
     return c
 
 
+def vanAlbada1(u, c, d):
+    """vanAlbada1 limiter, flow.jl:8-11."""
+    a, b = c - u, d - c
+    return c + max(a * b, 0) * (0 if a == b == 0 else (a + b) / (a * a + b * b)) / 2
+
+
+def Sweby(u, c, d, γ=1.5):
+    """Sweby limiter, flow.jl:12 (γ = 1.5; the kernels implement γ = 1.5 and γ = 2)."""
+    s = np.sign(d - u)
+    if c <= min(u, d) or c >= max(u, d):
+        return c
+    return c + s * max(0, min(s * γ * (c - u), s * (d - c)), min(s * (c - u), s * γ * (d - c))) / 2
+
+
+def superbee(u, c, d):
+    """superbee = Sweby(γ = 2), flow.jl:13."""
+    return Sweby(u, c, d, 2)
+
+
+def TVDcen(u, c, d):
+    """TVDcen, flow.jl:14."""
+    s = np.sign(d - u)
+    return c if (c <= min(u, d) or c >= max(u, d)) else c + s * min(s * (c - u), s * (d - c) / 2)
+
+
+def TVDdown(u, c, d):
+    """TVDdown, flow.jl:15."""
+    s = np.sign(d - u)
+    return c if (c <= min(u, d) or c >= max(u, d)) else c + s * min(s * (c - u), s * (d - c))
+
+
 def getInterfaceNormal_WH(f, nhat, I):
     """Selector for the WH normal scheme (normalEstimation.jl:105)."""
     raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
@@ -136,8 +167,37 @@ def getInterfaceNormal_Column(f, nhat, I):
     raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
 
 
-_LIMITERS = {Koren: 0, minmod: 1, upwind: 2, "Koren": 0, "minmod": 1, "upwind": 2}
-_SCHEMES = {getInterfaceNormal_WH: 0, getInterfaceNormal_Column: 1, "WH": 0, "Column": 1}
+def getInterfaceNormal_WY(f, nhat, I):
+    """Selector for the WY normal scheme (normalEstimation.jl:35)."""
+    raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
+
+
+def getInterfaceNormal_MYC(f, nhat, I):
+    """Selector for the MYC normal scheme (normalEstimation.jl:185)."""
+    raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
+
+
+def getInterfaceNormal_PCD(f, nhat, I):
+    """Selector for the PCD normal scheme (normalEstimation.jl:161)."""
+    raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
+
+
+def getInterfaceNormal_Y(f, nhat, I):
+    """Selector for the Youngs normal scheme (normalEstimation.jl:232)."""
+    raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
+
+
+def getInterfaceNormal_SLIC(f, nhat, I):
+    """Selector for the SLIC normal scheme (normalEstimation.jl:172)."""
+    raise TypeError("normal schemes run on the device; pass the function as normalScheme=")
+
+
+_LIMITERS = {Koren: 0, minmod: 1, upwind: 2, vanAlbada1: 3, superbee: 4, TVDcen: 5, TVDdown: 6, Sweby: 7,
+             "Koren": 0, "minmod": 1, "upwind": 2, "vanAlbada1": 3, "superbee": 4, "TVDcen": 5, "TVDdown": 6,
+             "Sweby": 7}
+_SCHEMES = {getInterfaceNormal_WH: 0, getInterfaceNormal_Column: 1, getInterfaceNormal_WY: 2,
+            getInterfaceNormal_MYC: 3, getInterfaceNormal_PCD: 4, getInterfaceNormal_Y: 5, getInterfaceNormal_SLIC: 6,
+            "WH": 0, "Column": 1, "WY": 2, "MYC": 3, "PCD": 4, "Youngs": 5, "SLIC": 6}
 
 
 def sdf_samples(dims, sdf, T):

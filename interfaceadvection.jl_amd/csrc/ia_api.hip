@@ -1230,7 +1230,8 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     for (int a = 0; a < p->D; ++a)
         if (p->N[a] < 2) return IA_ERR_BADARG;
     if (p->eta != 0) return IA_ERR_BADARG;  // surface tension is out of scope (η == nothing)
-    if (p->normal_scheme < 0 || p->normal_scheme > 1 || p->limiter < 0 || p->limiter > 2) return IA_ERR_BADARG;
+    if (p->normal_scheme < 0 || p->normal_scheme > IA_NORMAL_SLIC || p->limiter < 0 || p->limiter > IA_LIMITER_SWEBY)
+        return IA_ERR_BADARG;
     if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PLAIN) return IA_ERR_BADARG;
     if (p->poisson_mode != IA_POISSON_REFERENCE && p->dtype != IA_F32) return IA_ERR_BADARG;
     if (p->poisson_mode == IA_POISSON_LEARNED &&

@@ -329,17 +329,126 @@ template <class T> __device__ inline void normal_wh(const Grid &g, const T *f, l
         }
     }
 }
+// normalEstimation.jl:35-68 (WY)
+template <class T> __device__ inline void normal_wy(const Grid &g, const T *f, long I, T *n) {
+    for (int d = 0; d < g.D; ++d) n[d] = f[I - g.st[d]] - f[I + g.st[d]];
+    int dom = argabsmax(g.D, n);
+    for (int d = 0; d < g.D; ++d) {
+        if (d == dom) {
+            T s = tsign(n[d]);
+            n[d] = s == T(0) ? T(1) : s;
+            continue;
+        }
+        T hl = h3(g, f, I - g.st[d], dom), hc = h3(g, f, I, dom), hr = h3(g, f, I + g.st[d], dom);
+        T v = (hl - hr) / 2;
+        if (fabs(v) > T(0.5)) v = ((double)v * ((double)hc - 1.5) >= 0.0) ? hc - hr : hl - hc;
+        n[d] = v;
+    }
+}
+// normalEstimation.jl:248-257 YoungSum: the 2^(D-1) cells II ∈ I-δxy:I and, for
+// each, III ∈ II:II+δxy (δxy = 1 on every axis but d), in column-major order
+template <class T> __device__ inline T young_sum(const Grid &g, const T *f, long I, int d) {
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int a = 0; a < g.D; ++a)
+        if (a != d) lo[a] = -1;
+    T s = 0;
+    for (int c2 = lo[2]; c2 <= hi[2]; ++c2)
+        for (int c1 = lo[1]; c1 <= hi[1]; ++c1)
+            for (int c0 = lo[0]; c0 <= hi[0]; ++c0) {
+                long II = I + c0 + g.st[1] * c1 + g.st[2] * c2;
+                int h2[3] = {0, 0, 0};
+                for (int a = 0; a < g.D; ++a)
+                    if (a != d) h2[a] = 1;
+                for (int e2 = 0; e2 <= h2[2]; ++e2)
+                    for (int e1 = 0; e1 <= h2[1]; ++e1)
+                        for (int e0 = 0; e0 <= h2[0]; ++e0) s += f[II + e0 + g.st[1] * e1 + g.st[2] * e2];
+            }
+    return s;
+}
+// normalEstimation.jl:232-247 (Youngs)
+template <class T> __device__ inline void normal_youngs(const Grid &g, const T *f, long I, T *n) {
+    T a = 0;
+    for (int d = 0; d < g.D; ++d) {
+        n[d] = (T)((double)(young_sum(g, f, I - g.st[d], d) - young_sum(g, f, I + g.st[d], d)) * 0.5);
+        a += fabs(n[d]);
+    }
+    for (int d = 0; d < g.D; ++d) n[d] = a == T(0) ? (T)(1.0 / g.D) : n[d] / a;
+}
+// normalEstimation.jl:210-224 (CCi with dominant axis dc; signs from the Youngs normal ny)
+template <class T> __device__ inline void normal_cci(const Grid &g, const T *f, long I, const T *ny, int dc, T *out) {
+    T s = 0;
+    for (int d = 0; d < g.D; ++d) {
+        if (d == dc) {
+            T sg = tsign(ny[d]);
+            out[d] = sg == T(0) ? T(1) : sg;
+        } else {
+            T hu = h3(g, f, I + g.st[d], dc), hd = h3(g, f, I - g.st[d], dc);
+            out[d] = -(hu - hd) / 2;
+        }
+    }
+    for (int d = 0; d < g.D; ++d) s += fabs(out[d]);
+    for (int d = 0; d < g.D; ++d) out[d] /= s;
+}
+// normalEstimation.jl:185-202 (MYC): Youngs, replaced by the CCi normal of the
+// axis picked by the scan below when that one's dominant component is smaller
+template <class T> __device__ inline void normal_myc(const Grid &g, const T *f, long I, T *n) {
+    normal_youngs(g, f, I, n);
+    T mx = 0;
+    for (int i = 0; i < g.D; ++i) mx = fabs(n[i]) > mx ? fabs(n[i]) : mx;
+    T curm0 = 0, cc[3];
+    int ccz = 0;
+    for (int iz = 0; iz < g.D; ++iz) {
+        normal_cci(g, f, I, n, iz, cc);
+        if (fabs(cc[iz]) > curm0) ccz = iz;
+        curm0 = fabs(cc[iz]);
+    }
+    normal_cci(g, f, I, n, ccz, cc);
+    if (fabs(cc[ccz]) < mx)
+        for (int i = 0; i < g.D; ++i) n[i] = cc[i];
+}
+// normalEstimation.jl:161-165 (PCD) and :172-176 (SLIC: the dominant PCD axis only)
+template <class T> __device__ inline void normal_pcd(const Grid &g, const T *f, long I, T *n) {
+    for (int d = 0; d < g.D; ++d) n[d] = f[I - g.st[d]] - f[I + g.st[d]];
+}
+template <class T> __device__ inline void normal_slic(const Grid &g, const T *f, long I, T *n) {
+    normal_pcd(g, f, I, n);
+    int dom = argabsmax(g.D, n);
+    for (int i = 0; i < g.D; ++i) n[i] = i == dom ? tsign(n[i]) : T(0);
+}
 template <class T> __device__ inline void normal_scheme(int scheme, const Grid &g, const T *f, long I, T *n) {
     if (scheme == 1) normal_column(g, f, I, n);
+    else if (scheme == 2) normal_wy(g, f, I, n);
+    else if (scheme == 3) normal_myc(g, f, I, n);
+    else if (scheme == 4) normal_pcd(g, f, I, n);
+    else if (scheme == 5) normal_youngs(g, f, I, n);
+    else if (scheme == 6) normal_slic(g, f, I, n);
     else normal_wh(g, f, I, n);
 }
 
 // ---------------------------------------------------------------------------
-// limiters (flow.jl:5-7)
+// limiters (flow.jl:5-15)
 // ---------------------------------------------------------------------------
 template <class T> __device__ inline T limiter(int lim, T u, T c, T d) {
     if (lim == 1) return median3((3 * c - u) / 2, c, (c + d) / 2);  // minmod
     if (lim == 2) return c;                                         // upwind
+    if (lim == 3) {                                                 // vanAlbada1
+        T al = c - u, be = d - c;
+        T w = (al == be && al == T(0)) ? T(0) : (al + be) / (al * al + be * be);
+        return c + (tmax(al * be, T(0)) * w) / 2;
+    }
+    if (lim == 4) {  // superbee = Sweby(γ = 2)
+        T s = tsign(d - u), gm = 2;
+        if (c <= tmin(u, d) || c >= tmax(u, d)) return c;
+        return c + (s * tmax(tmax(T(0), tmin((s * gm) * (c - u), s * (d - c))), tmin(s * (c - u), (s * gm) * (d - c)))) / 2;
+    }
+    if (lim == 5 || lim == 6 || lim == 7) {  // TVDcen, TVDdown, Sweby(γ = 1.5)
+        T s = tsign(d - u);
+        if (c <= tmin(u, d) || c >= tmax(u, d)) return c;
+        if (lim == 5) return c + s * tmin(s * (c - u), (s * (d - c)) / 2);
+        if (lim == 6) return c + s * tmin(s * (c - u), s * (d - c));
+        T gm = T(1.5);
+        return c + (s * tmax(tmax(T(0), tmin((s * gm) * (c - u), s * (d - c))), tmin(s * (c - u), (s * gm) * (d - c)))) / 2;
+    }
     return median3(((7 * c + d) - 2 * u) / 6, c, median3(2 * c - u, c, d));  // Koren
 }
 

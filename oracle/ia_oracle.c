@@ -430,11 +430,20 @@ static void normal_myc(const grid_t *g, const T *f, T *nh, size_t I) {
     if (fabs(cc[ccz - 1]) < mx)
         for (int i = 1; i <= g->D; ++i) VC(nh, i)[I] = cc[i - 1];
 }
+/* normalEstimation.jl:172-176 (SLIC): the dominant PCD axis only */
+static void normal_slic(const grid_t *g, const T *f, T *nh, size_t I) {
+    normal_pcd(g, f, nh, I);
+    int d = argabsmax(g, nh, I);
+    for (int i = 1; i <= g->D; ++i) VC(nh, i)[I] = i == d ? tsign(VC(nh, i)[I]) : (T)0;
+}
 static void normal_scheme(const grid_t *g, int scheme, const T *f, T *nh, size_t I) {
     switch (scheme) {
     case 1: normal_column(g, f, nh, I); break;
     case 2: normal_wy(g, f, nh, I); break;
     case 3: normal_myc(g, f, nh, I); break;
+    case 4: normal_pcd(g, f, nh, I); break;    /* normalEstimation.jl:161-165 */
+    case 5: normal_youngs(g, f, nh, I); break; /* normalEstimation.jl:232-247 */
+    case 6: normal_slic(g, f, nh, I); break;
     default: normal_wh(g, f, nh, I); break;
     }
 }
@@ -944,6 +953,21 @@ static T limiter(int lim, T u, T c, T d) {
     }
     case 4: {                                                                /* superbee = Sweby(γ=2) */
         T s = tsign(d - u), gm = 2;
+        if (c <= tmin(u, d) || c >= tmax(u, d)) return c;
+        return c + (s * tmax(tmax(0, tmin((s * gm) * (c - u), s * (d - c))), tmin(s * (c - u), (s * gm) * (d - c)))) / 2;
+    }
+    case 5: {                                                                /* TVDcen */
+        T s = tsign(d - u);
+        if (c <= tmin(u, d) || c >= tmax(u, d)) return c;
+        return c + s * tmin(s * (c - u), (s * (d - c)) / 2);
+    }
+    case 6: {                                                                /* TVDdown */
+        T s = tsign(d - u);
+        if (c <= tmin(u, d) || c >= tmax(u, d)) return c;
+        return c + s * tmin(s * (c - u), s * (d - c));
+    }
+    case 7: {                                                                /* Sweby, γ = 1.5 */
+        T s = tsign(d - u), gm = (T)1.5;
         if (c <= tmin(u, d) || c >= tmax(u, d)) return c;
         return c + (s * tmax(tmax(0, tmin((s * gm) * (c - u), s * (d - c))), tmin(s * (c - u), (s * gm) * (d - c)))) / 2;
     }
@@ -1711,6 +1735,7 @@ real_t oc_getVolumeFraction3(real_t n1, real_t n2, real_t n3, real_t b) { return
 real_t oc_f2alpha3(real_t m1, real_t m2, real_t m3, real_t v) { return f2a3(m1, m2, m3, v); }
 real_t oc_alpha2f3(real_t m1, real_t m2, real_t m3, real_t a) { return a2f3(m1, m2, m3, a); }
 real_t oc_koren(real_t u, real_t c, real_t d) { return limiter(0, u, c, d); }
+real_t oc_limiter(int lim, real_t u, real_t c, real_t d) { return limiter(lim, u, c, d); }
 int oc_normal_block(int D, const int *Ng, const real_t *f, int scheme, const int *I, real_t *n_out) {
     grid_t gg, *g = &gg;
     grid_init(g, D, Ng, 0);

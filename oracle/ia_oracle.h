@@ -36,8 +36,8 @@ typedef struct oc_params {
     double nu;          /* flow.ν (= μ of the dark fluid, ρ_dark = 1); 0 -> no viscosity */
     double lambda_rho;  /* λρ */
     double lambda_mu;   /* λμ */
-    int normal_scheme;  /* 0 WH (default), 1 Column, 2 WY, 3 MYC */
-    int limiter;        /* 0 Koren (default), 1 minmod, 2 upwind, 3 vanAlbada1, 4 superbee */
+    int normal_scheme;  /* 0 WH (default), 1 Column, 2 WY, 3 MYC, 4 PCD, 5 Youngs, 6 SLIC */
+    int limiter;        /* 0 Koren (default), 1 minmod, 2 upwind, 3 vanAlbada1, 4 superbee, 5 TVDcen, 6 TVDdown, 7 Sweby */
     double dt0;         /* initial Δt of the Simulation (WaterLily default 0.25) */
     int mg_itmx;        /* solver! itmx (reference: 200) */
     double mg_tol;      /* solver! tol on Σr² (reference: 1e-4) */
@@ -108,6 +108,8 @@ int oc_f2face(int D, const int *Ng, const real_t *fc, real_t *ff, int perdir);
 int oc_vof_flux_face(int D, const int *Ng, real_t *ff, const real_t *f, const real_t *alpha,
                      const real_t *nhat, real_t dl, int d, const int *IFace, real_t *rhouf, real_t lr);
 real_t oc_koren(real_t u, real_t c, real_t d);
+/* flow.jl:5-15: 0 Koren, 1 minmod, 2 upwind, 3 vanAlbada1, 4 superbee, 5 TVDcen, 6 TVDdown, 7 Sweby(γ=1.5) */
+real_t oc_limiter(int lim, real_t u, real_t c, real_t d);
 void oc_set_threads(int n);
 const char *oc_last_error(oc_ctx *c);
 

@@ -19,8 +19,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 F, F0, ALPHA, NHAT, FFLUX, CBAR, RHOU, RHOUF, DRHO, U, U0, FORCE, P, SIGMA, MU0, MG_D, MG_ID, MG_R, MG_EPS = range(19)
 _VECTOR = {NHAT, RHOU, RHOUF, DRHO, U, U0, FORCE, MU0}
-NORMAL = {"WH": 0, "Column": 1, "WY": 2, "MYC": 3}
-LIMITER = {"Koren": 0, "minmod": 1, "upwind": 2, "vanAlbada1": 3, "superbee": 4}
+NORMAL = {"WH": 0, "Column": 1, "WY": 2, "MYC": 3, "PCD": 4, "Youngs": 5, "SLIC": 6}
+LIMITER = {"Koren": 0, "minmod": 1, "upwind": 2, "vanAlbada1": 3, "superbee": 4, "TVDcen": 5, "TVDdown": 6,
+           "Sweby": 7}
 
 
 class OcParams(C.Structure):
@@ -79,6 +80,7 @@ def lib(T):
         "oc_f2alpha3": (R, [R, R, R, R]),
         "oc_alpha2f3": (R, [R, R, R, R]),
         "oc_koren": (R, [R, R, R]),
+        "oc_limiter": (R, [C.c_int, R, R, R]),
         "oc_normal_block": (C.c_int, [C.c_int, ip, vp, C.c_int, ip, vp]),
         "oc_getmu": (R, [C.c_int, ip, vp, C.c_int, C.c_int, ip, R, R, R]),
         "oc_bc_scalar": (C.c_int, [C.c_int, ip, vp, C.c_int, C.c_int]),

@@ -413,3 +413,57 @@ def test_nan_is_reported():
         ia.sim_step_(g)
     assert e.value.code == 1
     g.close()
+
+
+# --------------------------------------------------------------------------- plug-ins
+def _pair_plugin(case, scheme="WH", lim="Koren"):
+    """GPU and oracle with one plug-in; the pressure solves run to round-off
+    (Σr² tolerance 1e-20, 200 V-cycles) so the velocities compare tightly."""
+    kw = dict(case["kw"])
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], normalScheme=scheme, λ=lim, poisson_tol=1e-20,
+                              **kw)
+    o = O.OracleSim(case["dims"], case["u_BC"], case["L"], normal=scheme, limiter=lim, mg_tol=1e-20, **kw)
+    return g, o
+
+
+@pytest.mark.parametrize("scheme", ["Column", "WY", "MYC", "PCD", "Youngs", "SLIC"])
+@pytest.mark.parametrize("D", [2, 3])
+def test_reconstruct_normal_schemes(scheme, D):
+    """normalEstimation.jl:35-68 (WY), :75-97 (Column), :161-176 (PCD, SLIC),
+    :185-257 (MYC, Youngs) on the GPU."""
+    say("reconstructInterface!", scheme, f"{D}D FP64")
+    case = bubble2d(32, np.float64) if D == 2 else bubbles3d(16, np.float64)
+    case["kw"]["perdir"] = (1,)
+    g, o = _pair_plugin(case, scheme=scheme)
+    g._call("ia_reconstruct")
+    o.lib.oc_reconstruct(o.ctx)
+    assert maxdiff(O.inside(g.intf.alpha), O.inside(o.get(O.ALPHA))) <= 1e-12
+    assert maxdiff(O.inside(g.intf.nhat, D), O.inside(o.get(O.NHAT), D)) <= 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("scheme", ["WY", "MYC", "Youngs", "SLIC"])
+def test_steps_with_normal_schemes(scheme):
+    say("free surface + bubble 32^2 FP64, 3 steps, normal", scheme)
+    g, o = _pair_plugin(bubble2d(32, np.float64), scheme=scheme)
+    for n in range(3):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= 1e-11, n
+        # solves to round-off whose dot products sum in another order: 1e-9 of max|u|
+        assert maxdiff(g.flow.u, o.u) <= 1e-9 * max(1e-12, np.abs(o.u).max()), n
+    g.close()
+
+
+@pytest.mark.parametrize("lim", ["minmod", "upwind", "vanAlbada1", "superbee", "TVDcen", "TVDdown", "Sweby"])
+def test_steps_with_limiters(lim):
+    """flow.jl:5-15 plug-ins in the SynDRoM face values of the momentum sweep."""
+    say("free surface + bubble 32^2 FP64, 3 steps, limiter", lim)
+    g, o = _pair_plugin(bubble2d(32, np.float64), lim=lim)
+    for n in range(3):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= 1e-11, n
+        # solves to round-off whose dot products sum in another order: 1e-9 of max|u|
+        assert maxdiff(g.flow.u, o.u) <= 1e-9 * max(1e-12, np.abs(o.u).max()), n
+    g.close()

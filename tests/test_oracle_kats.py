@@ -180,6 +180,24 @@ def test_normal_kats():
     assert out[0] == 0.5 and out[1] == 0.5
 
 
+def test_normal_pcd_youngs_slic_closed_forms():
+    """normalEstimation.jl:161-176 (PCD, SLIC) and :232-257 (Youngs) on f linear in x:
+    f = 0.2, 0.5, 0.8 along x, constant along y.  PCD = f[I-δ] - f[I+δ] = (-0.6, 0);
+    SLIC keeps the sign of the dominant PCD axis = (-1, 0); Youngs sums 2x2 stencils
+    either side (0.8 and 3.2 along x, 2.0 and 2.0 along y) -> (-1.2, 0) / 1.2."""
+    L = O.lib(np.float64)
+    Ng = O._ints([3, 3])
+    I = O._ints([2, 2])
+    out = np.zeros(2)
+    f = _blk([[0.2, 0.2, 0.2], [0.5, 0.5, 0.5], [0.8, 0.8, 0.8]])
+    L.oc_normal_block(2, Ng, O._ptr(f), O.NORMAL["PCD"], I, O._ptr(out))
+    assert abs(out[0] + 0.6) < 1e-15 and out[1] == 0
+    L.oc_normal_block(2, Ng, O._ptr(f), O.NORMAL["SLIC"], I, O._ptr(out))
+    assert out[0] == -1 and out[1] == 0
+    L.oc_normal_block(2, Ng, O._ptr(f), O.NORMAL["Youngs"], I, O._ptr(out))
+    assert abs(out[0] + 1) < 1e-15 and out[1] == 0
+
+
 def test_vof_flux_kats():
     """maintests.jl:176-198 (getVOFFlux! per face)."""
     L = O.lib(np.float64)

@@ -273,3 +273,29 @@ def test_vof_advect_conserves_volume(T, tol):
     f = sim.f
     assert f.min() >= 0 and f.max() <= 1
     assert abs(O.inside(f).astype(np.float64).sum() - V0) < tol * V0
+
+
+# --------------------------------------------------------------------------- limiters (flow.jl:5-15)
+# hand-computed values: monotone data (u, c, d) = (0, 1, 3) and (3, 2, 0.5), and
+# the extremum case (0, 2, 1) where every TVD limiter returns c
+LIMITER_KAT = {
+    # (0, 1, 3)   (3, 2, 0.5)     (0, 2, 1)
+    0: (10 / 6, 8.5 / 6, 2.0),           # Koren: median((7c+d-2u)/6, c, median(2c-u, c, d))
+    1: (1.5, 1.5, 2.0),                  # minmod: median((3c-u)/2, c, (c+d)/2)
+    2: (1.0, 2.0, 2.0),                  # upwind: c
+    3: (1.6, 2 - 1.5 * (2.5 / 3.25) / 2, 2.0),  # vanAlbada1: c + max(αβ,0)(α+β)/(α²+β²)/2
+    4: (2.0, 1.25, 2.0),                 # superbee = Sweby(γ=2)
+    5: (2.0, 1.25, 2.0),                 # TVDcen: c + s min(s(c-u), s(d-c)/2)
+    6: (2.0, 1.0, 2.0),                  # TVDdown: c + s min(s(c-u), s(d-c))
+    7: (1.75, 1.25, 2.0),                # Sweby(γ=1.5)
+}
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_limiter_closed_forms(T):
+    lib = O.lib(T)
+    cases = [(0.0, 1.0, 3.0), (3.0, 2.0, 0.5), (0.0, 2.0, 1.0)]
+    for lim, want in LIMITER_KAT.items():
+        for (u, c, d), w in zip(cases, want):
+            got = lib.oc_limiter(lim, u, c, d)
+            assert abs(got - w) <= (4e-16 if T == np.float64 else 2e-7) * max(1.0, abs(w)), (lim, u, c, d, got, w)
