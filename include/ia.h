@@ -97,7 +97,7 @@ typedef struct ia_params {
     double nu;             /* Flow.ν; cVOF.μ = ν (0 -> inviscid, cVOF.jl:78) */
     double lambda_rho;     /* λρ light/dark density ratio */
     double lambda_mu;      /* λμ light/dark viscosity ratio */
-    double eta;            /* surface tension; must be 0 (η == nothing) */
+    double eta;            /* surface tension coefficient η (surfaceTension.jl); 0 -> η === nothing */
     int normal_scheme;     /* enum ia_normal_scheme */
     int limiter;           /* enum ia_limiter */
     int poisson_mode;      /* enum ia_poisson_mode */

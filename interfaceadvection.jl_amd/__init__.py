@@ -268,8 +268,8 @@ class TwoPhaseSimulation:
             raise ValueError("exitBC is not supported by this backend")
         if body is not None:
             raise ValueError("immersed bodies are not supported by MPFMomStep! (InterfaceAdvection.jl:97)")
-        if η not in (None, 0, 0.0):
-            raise ValueError("surface tension (η != 0) is not supported by this backend")
+        if η is not None and not (float(η) > 0):
+            raise ValueError("surface tension coefficient η must be positive (or None)")
         self.dims = tuple(int(n) for n in dims)
         self.D = len(self.dims)
         self.Ng = tuple(n + 2 for n in self.dims)
@@ -292,7 +292,7 @@ class TwoPhaseSimulation:
         p.nu = float(ν)
         p.lambda_rho = float(λρ)
         p.lambda_mu = float(λμ)
-        p.eta = 0.0
+        p.eta = 0.0 if η is None else float(η)
         p.normal_scheme = _SCHEMES[normalScheme]
         p.limiter = _LIMITERS[λ]
         p.poisson_mode = POISSON[poisson]

@@ -79,6 +79,7 @@ struct ia_ctx {
         int projections = 0, accepted = 0;
         double last_ratio = 0;
     } cnn;
+    void *st_buf = nullptr, *st_fb = nullptr;  // surface tension forcing (vector) and face-averaged f
     const int *halt = nullptr;   // the flexible CG's stop flag while a pipelined solve runs
     FcgState *pin = nullptr;     // pinned host copies of the solver state (two slots)
     hipEvent_t it_ev[2] = {nullptr, nullptr};
@@ -942,7 +943,8 @@ template <class T> struct Impl {
             dtG = T(1) / (2 * (T)sqrt(g2));
         }
         T dtVisc = c->P.nu != 0 ? T(3) / ((14 * nu) * std::max(T(1), lm / lr)) : T(1);
-        T m = std::min(std::min(std::min(std::min(dtV, dtAdv), dtG), dtVisc), T(1));
+        T dtST = c->P.eta != 0 ? (T)std::sqrt((1 + lr) / ((T)(8 * M_PI) * (T)c->P.eta)) : T(1);
+        T m = std::min(std::min(std::min(std::min(dtV, dtAdv), dtG), dtVisc), dtST);
         *out = (double)(T(0.8) * m);
         return IA_OK;
     }
@@ -1039,6 +1041,24 @@ template <class T> struct Impl {
         }
         return cur;
     }
+    // surfTen! (surfaceTension.jl:7-19) on f into `out`: written (acc = 0) or added (acc = 1)
+    static int surften(ia_ctx *c, const void *f, void *out, int acc) {
+        Grid &g = c->g;
+        if (!c->st_fb) {
+            HIPCHK(c, hipMalloc(&c->st_fb, g.ns * sizeof(T)));
+            HIPCHK(c, hipMemsetAsync(c->st_fb, 0, g.ns * sizeof(T), c->s));
+        }
+        for (int d = 0; d < g.D; ++d) {
+            hipLaunchKernelGGL(k_st_fbuf<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>((void *)f), P<T>(c->st_fb), d);
+            for (int j = 0; j < g.D; ++j)
+                hipLaunchKernelGGL(k_st_bc<T>, blocks1(g.ns / g.Ng[j]), dim3(NT), 0, c->s, g, P<T>(c->st_fb), d, j);
+            auto ks = g.D == 3 ? k_surften<3, T> : k_surften<2, T>;
+            hipLaunchKernelGGL(ks, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(c->st_fb), P<T>((void *)f),
+                               (T)c->P.eta, d, P<T>(out), acc);
+        }
+        LAUNCHCHK(c);
+        return IA_OK;
+    }
     static int upd(ia_ctx *c, const void *ust, const void *u0, const void *fcur, const void *fupd, void *uout, T dt,
                    T w) {
         Grid &g = c->g;
@@ -1063,6 +1083,12 @@ template <class T> struct Impl {
         for (int q = 0; q < 3; ++q) a.g[q] = c->P.has_g ? (T)c->P.g[q] : T(0);
         a.v.ff = P<T>(c->nh);
         if (a.v.has_mu) hipLaunchKernelGGL(k_fface<T>, blocks1(g.ns), dim3(NT), 0, c->s, a.v, P<T>(c->nh));
+        a.st = nullptr;
+        if (c->P.eta != 0) {
+            if (!c->st_buf) HIPCHK(c, hipMalloc(&c->st_buf, g.ns * g.D * sizeof(T)));
+            if (int e = surften(c, fupd, c->st_buf, 0)) return e;
+            a.st = P<T>(c->st_buf);
+        }
         prof_begin(c, IA_PROF_VISC_UPDATE);
         auto kv = g.D == 3 ? k_visc_update<3, T> : k_visc_update<2, T>;
         hipLaunchKernelGGL(kv, blocks(c->t), dim3(NT), 0, c->s, a);
@@ -1166,6 +1192,8 @@ template <class T> struct Impl {
         hipLaunchKernelGGL(k_fface<T>, blocks1(c->g.ns), dim3(NT), 0, c->s, a, P<T>(c->nh));
         auto kf = c->g.D == 3 ? k_visc_force<3, T> : k_visc_force<2, T>;
         hipLaunchKernelGGL(kf, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->F));
+        if (c->P.eta != 0)
+            if (int e = surften(c, a.fv, c->F, 1)) return e;
         LAUNCHCHK(c);
         return IA_OK;
     }
@@ -1229,7 +1257,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     if (!(p->D == 2 || p->D == 3) || !(p->dtype == IA_F32 || p->dtype == IA_F64)) return IA_ERR_BADARG;
     for (int a = 0; a < p->D; ++a)
         if (p->N[a] < 2) return IA_ERR_BADARG;
-    if (p->eta != 0) return IA_ERR_BADARG;  // surface tension is out of scope (η == nothing)
+    if (!(p->eta >= 0)) return IA_ERR_BADARG;  // η = 0 means η === nothing
     if (p->normal_scheme < 0 || p->normal_scheme > IA_NORMAL_SLIC || p->limiter < 0 || p->limiter > IA_LIMITER_SWEBY)
         return IA_ERR_BADARG;
     if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PLAIN) return IA_ERR_BADARG;
@@ -1271,6 +1299,7 @@ void ia_destroy(ia_ctx *c) {
     (void)hipSetDevice(c->dev);
     if (c->s) (void)hipStreamSynchronize(c->s);
     void *bufs[] = {c->fb[0], c->fb[1], c->fb[2], c->fb[3], c->u, c->u0, c->ru, c->ru2, c->ruf, c->F, c->p, c->sig,
+                    c->st_buf, c->st_fb,
                     c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->pst, c->fst, c->pp, c->dsc, c->x64, c->r64, c->z64,
                     c->pa64, c->pb64, c->q64};
     for (void *b : bufs)

@@ -426,6 +426,136 @@ template <int D, class T> __device__ inline T visc_force(const ViscArgs<T> &a, i
     return r;
 }
 
+// ---------------------------------------------------------------------------
+// surface tension (surfaceTension.jl:7-124), per momentum direction d
+// ---------------------------------------------------------------------------
+template <class T> __device__ inline bool contain_if(T v) { return v > T(0) && v < T(1); }  // VOFutil.jl:144
+// getPopinetHeightAdaptive(I, f, i, monotonic=true) about cell p (0-based,
+// ghost-inclusive) along the signed axis i (1-based): to the air side, then to
+// the water side; a walk stops at the array's end (validCI, util.jl:82)
+template <class T> __device__ inline T popinet_height(const Grid &g, const T *f, const int p[3], int i) {
+    int ax = abs(i) - 1, s = i > 0 ? 1 : -1;
+    long st = g.st[ax] * s;
+    long I0 = lin(g, p);
+    T fnow = f[I0], H = fnow - T(0.5);
+    bool fin = fnow < T(1);
+    int c = p[ax];
+    long In = I0;
+    while (!fin || contain_if(fnow)) {
+        c += s;
+        In += st;
+        if (c < 0 || c > g.Ng[ax] - 1) break;
+        T v = f[In];
+        fnow = v > fnow ? T(0) : v;
+        H += fnow;
+        if (contain_if(fnow)) fin = true;
+    }
+    fnow = f[I0];
+    fin = fnow > T(0);
+    c = p[ax];
+    In = I0;
+    while (!fin || contain_if(fnow)) {
+        c -= s;
+        In -= st;
+        if (c < 0 || c > g.Ng[ax] - 1) break;
+        T v = f[In];
+        fnow = v < fnow ? T(1) : v;
+        H += fnow - 1;
+        if (contain_if(fnow)) fin = true;
+    }
+    return H;
+}
+template <class T> __device__ inline T root1p5(T a) { return sqrt((a * a) * a); }  // surfaceTension.jl:89
+// getCurvature (surfaceTension.jl:35-80) with getXdir / getXYdir (util.jl:62-63)
+template <int D, class T> __device__ inline T curvature(const Grid &g, const T *f, const int p[3], int i) {
+    int s = i > 0 ? 1 : -1, ai = abs(i);
+    if (D == 2) {
+        int ix = ai == 1 ? -2 * s : s, ax = abs(ix) - 1, sx = ix > 0 ? 1 : -1;
+        T H[3];
+        for (int u = -1; u <= 1; ++u) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[ax] += u * sx;
+            H[u + 1] = popinet_height(g, f, q, i);
+        }
+        T Hx = (H[2] - H[0]) / 2;
+        T Hxx = (H[2] + H[0]) - 2 * H[1];
+        return Hxx / root1p5(1 + Hx * Hx);
+    }
+    int ix = s * (ai % 3 + 1), iy = (ai + 1) % 3 + 1;
+    int axx = abs(ix) - 1, sx = ix > 0 ? 1 : -1;
+    T H[3][3], flt = T(0.2);
+    for (int u = -1; u <= 1; ++u)
+        for (int v = -1; v <= 1; ++v) {
+            int q[3] = {p[0], p[1], p[2]};
+            q[axx] += u * sx;
+            q[iy - 1] += v;
+            H[u + 1][v + 1] = popinet_height(g, f, q, i);
+        }
+    T Hx = (H[2][1] - H[0][1]) / 2, Hy = (H[1][2] - H[1][0]) / 2;
+    T Hxx = (((H[2][1] + H[0][1]) - 2 * H[1][1]) + ((H[2][0] + H[0][0]) - 2 * H[1][0]) * flt +
+             ((H[2][2] + H[0][2]) - 2 * H[1][2]) * flt) / (1 + 2 * flt);
+    T Hyy = (((H[1][2] + H[1][0]) - 2 * H[1][1]) + ((H[0][2] + H[0][0]) - 2 * H[0][1]) * flt +
+             ((H[2][2] + H[2][0]) - 2 * H[2][1]) * flt) / (1 + 2 * flt);
+    T Hxy = (((H[2][2] + H[0][0]) - H[2][0]) - H[0][2]) / 4;
+    return ((Hxx * (1 + Hy * Hy) + Hyy * (1 + Hx * Hx)) - ((2 * Hxy) * Hx) * Hy) / root1p5((1 + Hx * Hx) + Hy * Hy);
+}
+// fbuffer[I] = ϕ(d, I, f) = (f[I] + f[I-δd]) / 2 on the interior
+template <class T> __global__ void __launch_bounds__(NT) k_st_fbuf(Grid g, Tiling t, const T *f, T *fb, int d) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    fb[I] = (f[I] + f[I - g.st[d]]) / 2;
+}
+// BCf!(d, fbuffer) on the ghost planes of axis j (VOFutil.jl:76-89), axis by
+// axis as the reference loops; the upper plane of axis d, which BCf!(d, ·)
+// does not write, is set to 0
+template <class T> __global__ void __launch_bounds__(NT) k_st_bc(Grid g, T *fb, int d, int j) {
+    long n = g.ns / g.Ng[j];
+    long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= n) return;
+    int c[3];
+    long r = q;
+    for (int a = 0; a < 3; ++a) {
+        if (a == j) continue;
+        c[a] = (int)(r % g.Ng[a]);
+        r /= g.Ng[a];
+    }
+    c[j] = 0;
+    long lo = lin(g, c), st = g.st[j], N = g.Ng[j];
+    long hi = lo + (N - 1) * st;
+    if (g.per[j]) {
+        fb[lo] = fb[lo + (N - 2) * st];
+        fb[hi] = fb[lo + st];
+    } else if (j == d) {
+        fb[lo] = fb[lo + 2 * st];
+        fb[hi] = T(0);
+    } else {
+        fb[lo] = fb[lo + st];
+        fb[hi] = fb[hi - st];
+    }
+}
+// forcing[I,d] (+)= η κ(I, fbuffer, majorDir) (-∂_d f) at interface cells of
+// fbuffer, 0 elsewhere (surfaceTension.jl:7-19)
+template <int D, class T>
+__global__ void __launch_bounds__(NT) k_surften(Grid g, Tiling t, const T *fb, const T *f, T eta, int d, T *out,
+                                                int acc) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T st = 0;
+    bool hit = contain_if(fb[I]);
+    if (hit) {
+        T n[3];
+        normal_wy(g, fb, I, n);
+        int im = argabsmax(g.D, n);
+        int i = (signbit(n[im]) ? -1 : 1) * (im + 1);
+        st = eta * curvature<D>(g, fb, p, i) * -(f[I] - f[I - g.st[d]]);
+    }
+    long o = g.ns * d + I;
+    if (!acc) out[o] = st;
+    else if (hit) out[o] = out[o] + st;
+}
+
 template <class T> struct UpdArgs {
     ViscArgs<T> v;
     const T *u0;      // Flow.u⁰ (for ρu⁰ = u⁰ρ(f_cur), flow.jl:76/:99)
@@ -434,6 +564,7 @@ template <class T> struct UpdArgs {
     const T *ru;      // ρu after advection
     T *uout;
     T *L;             // μ₀
+    const T *st;      // surface tension forcing (vector), or nullptr when η == nothing
     T dt, w, g[3];
 };
 template <int D, class T>
@@ -447,6 +578,7 @@ __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
     for (int i = 0; i < D; ++i) {
         long o = g.ns * i + I;
         T F = visc_force<D>(a.v, i, p);
+        if (a.st) F = F + a.st[o];
         T ru0 = a.u0[o] * linp((a.fcur[I] + a.fcur[I - g.st[i]]) / 2, a.v.lr);
         T run = ((ac * ru0 + a.ru[o]) + F * a.dt) * wT;
         T rho = linp((a.fupd[I] + a.fupd[I - g.st[i]]) / 2, a.v.lr);

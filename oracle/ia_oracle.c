@@ -580,7 +580,8 @@ struct oc_ctx {
     int D;
     T uBC[3], gv[3], zero3[3];
     int has_g, has_mu;
-    T nu, mu, lr, lm;
+    T nu, mu, lr, lm, eta;
+    int has_eta;
     int scheme, limiter;
     T *u, *u0, *fv, *p, *sig, *mu0;
     T *f, *f0, *al, *nh, *ff, *ru, *ruf, *dr;
@@ -1121,8 +1122,117 @@ static inline T viscF(const oc_ctx *c, int i, int jj, size_t I, const T *u, cons
     T mu = getmu(g, i, jj, I, ff, c->lm, c->mu, c->lr);
     return mu * ((VC(u, i)[I] - VC(u, i)[I - ST(jj)]) + (VC(u, jj)[I] - VC(u, jj)[I - ST(i)]));
 }
-/* flow.jl:113-139 viscSurfTenρu! / visc!  (surface tension is a no-op: η == nothing) */
+/* ------------------------------------------------------------------ */
+/* surfaceTension.jl                                                    */
+/* ------------------------------------------------------------------ */
+static inline int contain_interface(T v) { return v > 0 && v < 1; } /* VOFutil.jl:144 */
+/* surfaceTension.jl:96-124 getPopinetHeightAdaptive(I, f, i, monotonic=true):
+ * column height about cell q (1-based) along the signed axis i (1-based),
+ * walking to the air side, then to the water side; validCI (util.jl:82) stops
+ * a walk at the array's end */
+static T popinet_height(const grid_t *g, const T *f, const int q[3], int i) {
+    int ax = abs(i), s = i > 0 ? 1 : -1;
+    long st = (long)ST(ax) * s;
+    size_t I0 = IX(q[0], q[1], q[2]);
+    T fnow = f[I0], H = fnow - (T)0.5;
+    int fin = fnow < 1, c = q[ax - 1];
+    long In = (long)I0;
+    while (!fin || contain_interface(fnow)) {
+        c += s;
+        In += st;
+        if (c < 1 || c > NG(ax)) break;
+        T v = f[In];
+        fnow = v > fnow ? (T)0 : v;
+        H += fnow;
+        if (contain_interface(fnow)) fin = 1;
+    }
+    fnow = f[I0];
+    fin = fnow > 0;
+    c = q[ax - 1];
+    In = (long)I0;
+    while (!fin || contain_interface(fnow)) {
+        c -= s;
+        In -= st;
+        if (c < 1 || c > NG(ax)) break;
+        T v = f[In];
+        fnow = v < fnow ? (T)1 : v;
+        H += fnow - 1;
+        if (contain_interface(fnow)) fin = 1;
+    }
+    return H;
+}
+/* surfaceTension.jl:89 root1p5 */
+static inline T root1p5(T a) { return sqrt((a * a) * a); }
+/* surfaceTension.jl:35-80 getCurvature (util.jl:62-63 getXdir / getXYdir) */
+static T curvature(const grid_t *g, const T *f, const int q[3], int i) {
+    int s = i > 0 ? 1 : -1, ai = abs(i);
+    if (g->D == 2) {
+        int ix = ai == 1 ? -2 * s : s, ax = abs(ix), sx = ix > 0 ? 1 : -1;
+        T H[3];
+        for (int u = -1; u <= 1; ++u) {
+            int p[3] = {q[0], q[1], q[2]};
+            p[ax - 1] += u * sx;
+            H[u + 1] = popinet_height(g, f, p, i);
+        }
+        T Hx = (H[2] - H[0]) / 2;
+        T Hxx = (H[2] + H[0]) - 2 * H[1];
+        return Hxx / root1p5(1 + Hx * Hx);
+    }
+    int ix = s * (ai % 3 + 1), iy = (ai + 1) % 3 + 1;
+    int axx = abs(ix), sx = ix > 0 ? 1 : -1;
+    T H[3][3], flt = (T)0.2;
+    for (int u = -1; u <= 1; ++u)
+        for (int v = -1; v <= 1; ++v) {
+            int p[3] = {q[0], q[1], q[2]};
+            p[axx - 1] += u * sx;
+            p[iy - 1] += v;
+            H[u + 1][v + 1] = popinet_height(g, f, p, i);
+        }
+    T Hx = (H[2][1] - H[0][1]) / 2, Hy = (H[1][2] - H[1][0]) / 2;
+    T Hxx = (((H[2][1] + H[0][1]) - 2 * H[1][1]) + ((H[2][0] + H[0][0]) - 2 * H[1][0]) * flt +
+             ((H[2][2] + H[0][2]) - 2 * H[1][2]) * flt) / (1 + 2 * flt);
+    T Hyy = (((H[1][2] + H[1][0]) - 2 * H[1][1]) + ((H[0][2] + H[0][0]) - 2 * H[0][1]) * flt +
+             ((H[2][2] + H[2][0]) - 2 * H[2][1]) * flt) / (1 + 2 * flt);
+    T Hxy = (((H[2][2] + H[0][0]) - H[2][0]) - H[0][2]) / 4;
+    return ((Hxx * (1 + Hy * Hy) + Hyy * (1 + Hx * Hx)) - ((2 * Hxy) * Hx) * Hy) / root1p5((1 + Hx * Hx) + Hy * Hy);
+}
+/* surfaceTension.jl:7-19 surfTen!: per direction d, the face-averaged f in the
+ * fᶠ buffer with BCf!(d), the WY normal of it in n̂ at interface cells, and
+ * forcing[I,d] += η κ (-∂_d f).  BCf!(d) leaves the upper ghost plane of axis
+ * d unwritten (it holds whatever fᶠ held); here it is set to 0, which is what
+ * the advection sweeps leave there when the wall-normal velocity is 0. */
+static void surften(oc_ctx *c, T *r, const T *f) {
+    grid_t *g = &c->g;
+    rng_t R = r_in(g);
+    for (int d = 1; d <= g->D; ++d) {
+        T *fb = c->ff, *rd = VC(r, d);
+        FOR_R(R) {
+            size_t I = IX(i, j, k);
+            fb[I] = (f[I] + f[I - ST(d)]) / 2;
+        }
+        bc_f(g, fb, d);
+        if (!PER(d)) {
+            rng_t R2 = r_slice(g, NG(d), d, 1);
+            FOR_R(R2) { fb[IX(i, j, k)] = 0; }
+        }
+        FOR_R(R) {
+            size_t I = IX(i, j, k);
+            if (!contain_interface(fb[I])) continue;
+            normal_wy(g, fb, c->nh, I);
+            int im = argabsmax(g, c->nh, I);
+            int sgn = signbit(VC(c->nh, im)[I]) ? -1 : 1;
+            int q[3] = {i, j, k};
+            rd[I] += c->eta * curvature(g, fb, q, sgn * im) * -(f[I] - f[I - ST(d)]);
+        }
+    }
+}
+/* flow.jl:113-139 viscSurfTenρu! / visc!  (+ surfTen! when η is set) */
+static void visc_only(oc_ctx *c, T *r, const T *u, T *ffc, T *Phi, const T *f);
 static void visc(oc_ctx *c, T *r, const T *u, T *ffc, T *Phi, const T *f) {
+    visc_only(c, r, u, ffc, Phi, f);
+    if (c->has_eta) surften(c, r, f);
+}
+static void visc_only(oc_ctx *c, T *r, const T *u, T *ffc, T *Phi, const T *f) {
     grid_t *g = &c->g;
     memset(r, 0, g->ns * g->D * sizeof(T));
     if (!c->has_mu) return;
@@ -1212,7 +1322,8 @@ static T mpcfl(oc_ctx *c) {
         dtG = (T)1 / (2 * sqrt(g2));
     }
     T dtVisc = c->has_mu ? (T)3 / ((14 * c->mu) * tmax(1, c->lm / c->lr)) : (T)1;
-    T m = tmin(tmin(tmin(tmin(dtV, dtAdv), dtG), dtVisc), (T)1); /* Δt_SurfT = Δt_max (η == nothing) */
+    T dtST = c->has_eta ? (T)sqrt((1 + c->lr) / ((T)(8 * M_PI) * c->eta)) : (T)1;
+    T m = tmin(tmin(tmin(tmin(dtV, dtAdv), dtG), dtVisc), dtST);
     return (T)0.8 * m;
 }
 /* flow.jl:343-347 inproject!  (+ the FP64 refinement variant below) */
@@ -1331,6 +1442,8 @@ oc_ctx *oc_create(const oc_params *p) {
     c->mu = (T)p->nu;
     c->lr = (T)p->lambda_rho;
     c->lm = (T)p->lambda_mu;
+    c->has_eta = p->eta != 0;
+    c->eta = (T)p->eta;
     c->scheme = p->normal_scheme;
     c->limiter = p->limiter;
     c->mg_tol = p->mg_tol > 0 ? p->mg_tol : 1e-4;
@@ -1736,6 +1849,18 @@ real_t oc_f2alpha3(real_t m1, real_t m2, real_t m3, real_t v) { return f2a3(m1, 
 real_t oc_alpha2f3(real_t m1, real_t m2, real_t m3, real_t a) { return a2f3(m1, m2, m3, a); }
 real_t oc_koren(real_t u, real_t c, real_t d) { return limiter(0, u, c, d); }
 real_t oc_limiter(int lim, real_t u, real_t c, real_t d) { return limiter(lim, u, c, d); }
+real_t oc_popinet_height(int D, const int *Ng, const real_t *f, const int *I, int i) {
+    grid_t g;
+    grid_init(&g, D, Ng, 0);
+    int q[3] = {I[0], D > 1 ? I[1] : 1, D > 2 ? I[2] : 1};
+    return popinet_height(&g, f, q, i);
+}
+real_t oc_curvature(int D, const int *Ng, const real_t *f, const int *I, int i) {
+    grid_t g;
+    grid_init(&g, D, Ng, 0);
+    int q[3] = {I[0], D > 1 ? I[1] : 1, D > 2 ? I[2] : 1};
+    return curvature(&g, f, q, i);
+}
 int oc_normal_block(int D, const int *Ng, const real_t *f, int scheme, const int *I, real_t *n_out) {
     grid_t gg, *g = &gg;
     grid_init(g, D, Ng, 0);

@@ -33,6 +33,7 @@ typedef struct oc_params {
     double u_bc[3];     /* constant boundary velocity tuple u_BC */
     double g[3];        /* constant gravity g(i,x,t) = g[i] */
     int has_g;          /* 0 -> g === nothing */
+    double eta;         /* surface tension coefficient η; 0 -> η === nothing */
     double nu;          /* flow.ν (= μ of the dark fluid, ρ_dark = 1); 0 -> no viscosity */
     double lambda_rho;  /* λρ */
     double lambda_mu;   /* λμ */
@@ -110,6 +111,10 @@ int oc_vof_flux_face(int D, const int *Ng, real_t *ff, const real_t *f, const re
 real_t oc_koren(real_t u, real_t c, real_t d);
 /* flow.jl:5-15: 0 Koren, 1 minmod, 2 upwind, 3 vanAlbada1, 4 superbee, 5 TVDcen, 6 TVDdown, 7 Sweby(γ=1.5) */
 real_t oc_limiter(int lim, real_t u, real_t c, real_t d);
+/* surfaceTension.jl: height (getPopinetHeight) and curvature (getCurvature) about
+ * cell I (1-based) of a standalone block, along the signed axis i (1-based) */
+real_t oc_popinet_height(int D, const int *Ng, const real_t *f, const int *I, int i);
+real_t oc_curvature(int D, const int *Ng, const real_t *f, const int *I, int i);
 void oc_set_threads(int n);
 const char *oc_last_error(oc_ctx *c);
 

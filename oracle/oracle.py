@@ -26,7 +26,7 @@ LIMITER = {"Koren": 0, "minmod": 1, "upwind": 2, "vanAlbada1": 3, "superbee": 4,
 
 class OcParams(C.Structure):
     _fields_ = [("D", C.c_int), ("N", C.c_int * 3), ("perdir", C.c_int),
-                ("u_bc", C.c_double * 3), ("g", C.c_double * 3), ("has_g", C.c_int),
+                ("u_bc", C.c_double * 3), ("g", C.c_double * 3), ("has_g", C.c_int), ("eta", C.c_double),
                 ("nu", C.c_double), ("lambda_rho", C.c_double), ("lambda_mu", C.c_double),
                 ("normal_scheme", C.c_int), ("limiter", C.c_int), ("dt0", C.c_double),
                 ("mg_itmx", C.c_int), ("mg_tol", C.c_double), ("poisson_mode", C.c_int),
@@ -81,6 +81,8 @@ def lib(T):
         "oc_alpha2f3": (R, [R, R, R, R]),
         "oc_koren": (R, [R, R, R]),
         "oc_limiter": (R, [C.c_int, R, R, R]),
+        "oc_popinet_height": (R, [C.c_int, ip, vp, ip, C.c_int]),
+        "oc_curvature": (R, [C.c_int, ip, vp, ip, C.c_int]),
         "oc_normal_block": (C.c_int, [C.c_int, ip, vp, C.c_int, ip, vp]),
         "oc_getmu": (R, [C.c_int, ip, vp, C.c_int, C.c_int, ip, R, R, R]),
         "oc_bc_scalar": (C.c_int, [C.c_int, ip, vp, C.c_int, C.c_int]),
@@ -153,7 +155,7 @@ class OracleSim:
 
     def __init__(self, dims, u_BC, L=1.0, *, T=np.float32, λμ=1e-2, λρ=1e-3, ν=0.0, g=None,
                  perdir=(), Δt=0.25, InterfaceSDF=None, u0=None, normal="WH", limiter="Koren",
-                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=1):
+                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=1, η=None):
         self.T = np.dtype(T)
         self.lib = lib(self.T)
         self.dims = tuple(int(n) for n in dims)
@@ -172,6 +174,7 @@ class OracleSim:
         p.nu = float(ν)
         p.lambda_rho = float(λρ)
         p.lambda_mu = float(λμ)
+        p.eta = 0.0 if η is None else float(η)
         p.normal_scheme = NORMAL[normal]
         p.limiter = LIMITER[limiter]
         p.dt0 = float(Δt)

@@ -46,7 +46,7 @@ def test_params_struct_matches_header():
 def test_host_rejects_unsupported_plugins():
     ia = load_package()
     with pytest.raises(ValueError):
-        ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), η=1.0)
+        ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), η=-1.0)
     with pytest.raises(ValueError):
         ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), exitBC=True)
     with pytest.raises(KeyError):

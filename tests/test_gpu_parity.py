@@ -467,3 +467,57 @@ def test_steps_with_limiters(lim):
         # solves to round-off whose dot products sum in another order: 1e-9 of max|u|
         assert maxdiff(g.flow.u, o.u) <= 1e-9 * max(1e-12, np.abs(o.u).max()), n
     g.close()
+
+
+# --------------------------------------------------------------------------- surface tension
+def droplet(N=32, D=2, T=np.float64, η=0.05, per=()):
+    """An elliptic droplet (dark fluid inside) off the grid's symmetry lines, so
+    no interface cell has two equally dominant normal components (a tie would
+    let round-off pick the height direction); density and viscosity ratio 0.1,
+    surface tension η."""
+    c = (0.5 * N + 0.37, 0.5 * N - 0.21, 0.5 * N + 0.13)
+    ax = (0.27 * N, 0.22 * N, 0.25 * N)
+    sdf = lambda x: 0.24 * N * (np.sqrt(sum(((x[a] - c[a]) / ax[a]) ** 2 for a in range(D))) - 1)
+    return dict(dims=(N,) * D, u_BC=(0.0,) * D, L=float(N),
+                kw=dict(T=T, λρ=0.1, λμ=0.1, ν=0.01, η=η, InterfaceSDF=sdf, perdir=per))
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("D", [2, 3])
+def test_surface_tension_force(T, D):
+    """viscSurfTenρu! with η set (surfaceTension.jl:7-124): the force field of the
+    GPU's surfTen! (face-averaged f, WY normal, height-function curvature) plus
+    the viscous stress, against the oracle's."""
+    say("surfTen! + visc!", T.__name__, f"{D}D")
+    case = droplet(32 if D == 2 else 16, D, T)
+    g, o = pair(case)
+    u = _swirl(case, T, amp=0.2)
+    g.flow.u = u
+    o.set(O.U, u)
+    for use_f0 in (0, 1):
+        g._call("ia_visc", use_f0)
+        o.lib.oc_visc(o.ctx, use_f0)
+        Fg, Fo = O.inside(g.flow.force, D), O.inside(o.get(O.FORCE), D)
+        scale = max(1e-30, np.abs(Fo).max())
+        assert np.abs(Fo).max() > 0
+        assert maxdiff(Fg, Fo) <= (1e-12 if T == np.float64 else 1e-5) * scale
+    assert g.MPCFL() == o.cfl()
+    g.close()
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_droplet_steps_with_surface_tension(D):
+    """Whole sim_step!s of a droplet with surface tension against the oracle."""
+    say("droplet with surface tension, FP64, 3 steps", f"{D}D")
+    case = droplet(32 if D == 2 else 16, D, np.float64, per=tuple(range(1, D + 1)))
+    kw = dict(case["kw"])
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], poisson_tol=1e-20, **kw)
+    o = O.OracleSim(case["dims"], case["u_BC"], case["L"], mg_tol=1e-20, **kw)
+    for n in range(3):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= 1e-11, n
+        assert maxdiff(g.flow.u, o.u) <= 1e-9 * max(1e-12, np.abs(o.u).max()), n
+        assert g.flow.Δt[-1] == o.Δt[-1]
+    assert np.abs(g.flow.u).max() > 0  # the surface tension drives a flow
+    g.close()

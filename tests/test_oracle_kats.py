@@ -281,3 +281,15 @@ def test_twophase_ctor_kat():
     assert sim.L == 8.0 and sim.U == 1.0
     f = sim.f
     assert (O.inside(f)[:4, :] == 1).all() and (O.inside(f)[4:, :] == 0).all()
+
+
+def test_surface_tension_kats():
+    """maintests.jl:182-191: getPopinetHeight and the 2D getCurvature (rows are x)."""
+    L = O.lib(np.float64)
+    f = _blk([[0.3, 0.2, 0.1, 0.1, 0.2, 0.3, 0.0, 0.0],
+              [1.0, 1.0, 0.6, 0.5, 0.3, 0.2, 0.0, 0.0],
+              [0.0, 0.0, 0.0, 0.1, 0.0, 0.0, 0.0, 0.0]])
+    Ng = O._ints([3, 8])
+    H = [L.oc_popinet_height(2, Ng, O._ptr(f), O._ints([x, 5]), 2) for x in (1, 2, 3)]
+    assert H == [-0.3, -0.9, -1.4]
+    assert abs(L.oc_curvature(2, Ng, O._ptr(f), O._ints([2, 5]), 2) - 0.0672718547928328) < 1e-15

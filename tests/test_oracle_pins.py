@@ -299,3 +299,42 @@ def test_limiter_closed_forms(T):
         for (u, c, d), w in zip(cases, want):
             got = lib.oc_limiter(lim, u, c, d)
             assert abs(got - w) <= (4e-16 if T == np.float64 else 2e-7) * max(1.0, abs(w)), (lim, u, c, d, got, w)
+
+
+# --------------------------------------------------------------------------- curvature
+def _ball_fraction(N, c, R, D, near, sub=64):
+    """Cell fractions of a ball (D=3) or disc (D=2): cells within 3 of `near`
+    (integer cell coordinates; cells i span [i, i+1)) whose centre lies within
+    one cell of the surface are sub-sampled sub^D times; the others are 0 or 1
+    by their centre."""
+    idx = np.indices((N,) * D).reshape(D, -1).T
+    f = np.zeros(len(idx))
+    s = (np.arange(sub) + 0.5) / sub
+    grid = np.stack(np.meshgrid(*([s] * D), indexing="ij"), -1).reshape(-1, D)
+    for q, cell in enumerate(idx):
+        ctr = cell + 0.5
+        dist = np.sqrt(((ctr - c) ** 2).sum())
+        if np.abs(cell - near).max() <= 3 and abs(dist - R) < 1.0:
+            pts = cell + grid
+            f[q] = (((pts - c) ** 2).sum(1) <= R * R).mean()
+        else:
+            f[q] = 1.0 if dist <= R else 0.0
+    return np.asfortranarray(f.reshape((N,) * D))
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_curvature_of_a_ball(D):
+    """surfaceTension.jl:35-80: at the top of a disc (D=2) or ball (D=3) of radius
+    R = 10.3 cells, the height-function curvature along +y / +z is -(D-1)/R within 8%
+    (a dropped second-derivative term or a sign slip is off by 50% or more)."""
+    N, R = 32, 10.3
+    c = np.array((16.5, 16.5, 16.5)[:D])
+    top = int(c[D - 1] + R)  # the interface cell column through the pole, 0-based
+    near = np.array([int(c[a]) for a in range(D - 1)] + [top])
+    f = _ball_fraction(N, c, R, D, near, sub=256 if D == 2 else 40)
+    L = O.lib(np.float64)
+    Ng = O._ints([N] * D)
+    I = [int(c[a]) + 1 for a in range(D - 1)] + [top + 1]
+    assert 0 < f[tuple(i - 1 for i in I)] < 1
+    k = L.oc_curvature(D, Ng, O._ptr(f), O._ints(I), D)
+    assert abs(k + (D - 1) / R) <= 0.08 * (D - 1) / R, k
