@@ -172,6 +172,13 @@ int ia_mg_update(ia_ctx *c);
 /* myproject!(flow,pois,w) with dt_w = w*Δt, then BC!(u)  flow.jl:328-347, :82 */
 int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats);
 
+/* Diagnostics of metrics.jl:15-50 integrated over the interior, in FP64:
+ * out[0] = Σ ρkeI(I,u,f,λρ,U), out[1..3] = Σ ρuI(i,I,u,f,λρ,U), out[4] = Σ ρgh(I,g,f,λρ,StatWL),
+ * out[5] = Σ EnsI(I,ω) with ω_i = ∂_j u_k - ∂_k u_j by backward differences at
+ * the cell's lower edges (2D: ω = ∂_x v - ∂_y u), out[6] = Σ f.  U and statwl may
+ * be NULL (zero). */
+int ia_metrics(ia_ctx *c, const double *U, const double *statwl, double *out);
+
 /* Learned Poisson correction (no counterpart in the reference).  A five-layer
  * 3x3x3 convolutional network (3 -> 16 -> 16 -> 16 -> 16 -> 2 channels, ReLU
  * between layers), evaluated with BF16 matrix-core products and FP32

@@ -88,6 +88,7 @@ def lib():
         "ia_profile_enable": (C.c_int, [vp, C.c_int, C.c_int]),
         "ia_profile_read": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "ia_last_error": (C.c_char_p, [vp]),
+        "ia_metrics": (C.c_int, [vp, vp, vp, vp]),
         "ia_learned_init": (C.c_int, [vp, C.c_uint]),
         "ia_learned_set_weights": (C.c_int, [vp, vp, sz]),
         "ia_learned_get_weights": (C.c_int, [vp, vp, sz]),
@@ -386,6 +387,20 @@ class TwoPhaseSimulation:
     def sim_time(self):
         """WaterLily sim_time: Σ Δt[1:end-1] scaled by U/L is the caller's choice; raw time here."""
         return float(sum(self.flow.Δt[:-1]))
+
+    def metrics(self, U=None, StatWL=None):
+        """Domain integrals of the diagnostics of metrics.jl:15-50 (FP64): kinetic
+        energy Σρke, momentum Σρu, potential energy Σρgh, enstrophy and volume Σf."""
+        u = np.zeros(3)
+        w = np.zeros(3)
+        if U is not None:
+            u[:len(U)] = U
+        if StatWL is not None:
+            w[:len(StatWL)] = StatWL
+        out = np.zeros(7)
+        self._call("ia_metrics", _ptr(u), _ptr(w), _ptr(out))
+        return dict(kinetic_energy=out[0], momentum=tuple(out[1:1 + self.D]), potential_energy=out[4],
+                    enstrophy=out[5], volume=out[6])
 
     # ---- learned Poisson correction (no reference counterpart)
     def learned_init(self, seed=0):

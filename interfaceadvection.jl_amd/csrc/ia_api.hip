@@ -949,6 +949,30 @@ template <class T> struct Impl {
         return IA_OK;
     }
 
+    // metrics.jl integrals (Σρke, Σρu_i, Σρgh, Σ enstrophy, Σf)
+    static int metrics(ia_ctx *c, const double *U, const double *wl, double *out) {
+        Grid &g = c->g;
+        long nb = c->t.nblk;
+        double *part = nullptr;  // seven per-block partials (a diagnostic: allocated per call)
+        HIPCHK(c, hipMalloc((void **)&part, 7 * nb * sizeof(double)));
+        double gv[3] = {0, 0, 0};
+        if (c->P.has_g)
+            for (int d = 0; d < g.D; ++d) gv[d] = (double)(T)c->P.g[d];
+        auto km = g.D == 3 ? k_metrics<3, T> : k_metrics<2, T>;
+        hipLaunchKernelGGL(km, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(c->u), P<T>(c->fb[c->fi]),
+                           (T)c->P.lambda_rho, U[0], U[1], U[2], gv[0], gv[1], gv[2], wl[0], wl[1], wl[2], part);
+        std::vector<double> h(7 * nb);
+        hipError_t e = hipMemcpyAsync(h.data(), part, 7 * nb * sizeof(double), hipMemcpyDeviceToHost, c->s);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+        (void)hipFree(part);
+        HIPCHK(c, e);
+        for (int q = 0; q < 7; ++q) {
+            double s = 0;
+            for (long b = 0; b < nb; ++b) s += h[b * 7 + q];
+            out[q] = s;
+        }
+        return IA_OK;
+    }
     // ----------------------------------------------------------------- sweeps
     static void prepare(ia_ctx *c, const void *f, const void *usrc, bool do_ru) {
         hipLaunchKernelGGL(k_prepare<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(f), P<T>(usrc), P<T>(c->ru),
@@ -1416,6 +1440,12 @@ int ia_synchronize(ia_ctx *c) {
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return IA_OK;
+}
+int ia_metrics(ia_ctx *c, const double *U, const double *statwl, double *out) {
+    if (!c || !out) return IA_ERR_BADARG;
+    double z[3] = {0, 0, 0};
+    (void)hipSetDevice(c->dev);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::metrics(c, U ? U : z, statwl ? statwl : z, out); });
 }
 int ia_learned_init(ia_ctx *c, unsigned seed) {
     if (!c || c->P.dtype != IA_F32 || c->g.D != 3 || (c->g.Ng[0] - 2) % 16) return IA_ERR_BADARG;

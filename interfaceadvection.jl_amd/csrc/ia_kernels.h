@@ -427,6 +427,60 @@ template <int D, class T> __device__ inline T visc_force(const ViscArgs<T> &a, i
 }
 
 // ---------------------------------------------------------------------------
+// diagnostics (metrics.jl:15-50) integrated over the interior, in FP64:
+// Σρke, Σρu_i, Σρgh, Σ enstrophy (ω by backward differences at the cell's
+// lower edges, which EnsI averages), Σf.  Seven block partials per block.
+// ---------------------------------------------------------------------------
+template <int D, class T> __device__ inline double vort_edge(const Grid &g, const T *u, long I, int a) {
+    if (D == 2) {
+        return (double)((u[g.ns + I] - u[g.ns + I - g.st[0]]) - (u[I] - u[I - g.st[1]]));
+    }
+    int j = (a + 1) % 3, k = (a + 2) % 3;
+    return (double)((u[g.ns * k + I] - u[g.ns * k + I - g.st[j]]) - (u[g.ns * j + I] - u[g.ns * j + I - g.st[k]]));
+}
+template <int D, class T>
+__global__ void __launch_bounds__(NT) k_metrics(Grid g, Tiling t, const T *u, const T *f, T lr, double U0, double U1,
+                                                double U2, double g0, double g1, double g2, double w0, double w1,
+                                                double w2, double *part) {
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    int p[3];
+    if (thread_cell(g, t, p)) {
+        long I = lin(g, p);
+        double U[3] = {U0, U1, U2}, gv[3] = {g0, g1, g2}, wl[3] = {w0, w1, w2};
+        double rho = (double)linp(f[I], lr);
+        double ke = 0, pe = 0;
+        for (int i = 0; i < D; ++i) {
+            const T *ui = u + g.ns * i;
+            double a = (double)ui[I] - U[i], b = (double)ui[I + g.st[i]] - U[i];
+            ke += (a * a + b * b) * rho;
+            acc[1 + i] = 0.5 * (((double)ui[I] + (double)ui[I + g.st[i]]) - 2 * U[i]) * rho;
+            pe += gv[i] * ((p[i] - 0.5) - wl[i]);
+        }
+        acc[0] = 0.25 * ke;
+        acc[4] = -rho * pe;
+        double en = 0;
+        if (D == 2) {
+            double a = vort_edge<D>(g, u, I, 0), b = vort_edge<D>(g, u, I + g.st[0], 0),
+                   c = vort_edge<D>(g, u, I + g.st[1], 0), d = vort_edge<D>(g, u, I + g.st[0] + g.st[1], 0);
+            en = ((a * a + b * b) + c * c) + d * d;
+        } else {
+            for (int i = 0; i < 3; ++i) {
+                int ix = (i + 1) % 3, iy = (i + 2) % 3;
+                double a = vort_edge<D>(g, u, I, i), b = vort_edge<D>(g, u, I + g.st[ix], i),
+                       c = vort_edge<D>(g, u, I + g.st[iy], i), d = vort_edge<D>(g, u, I + g.st[ix] + g.st[iy], i);
+                en += ((a * a + b * b) + c * c) + d * d;
+            }
+        }
+        acc[5] = 0.5 * 0.25 * en;
+        acc[6] = (double)f[I];
+    }
+    for (int q = 0; q < 7; ++q) {
+        double s = block_sum(acc[q]);
+        if (threadIdx.x == 0) part[(long)blockIdx.x * 7 + q] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // surface tension (surfaceTension.jl:7-124), per momentum direction d
 // ---------------------------------------------------------------------------
 template <class T> __device__ inline bool contain_if(T v) { return v > T(0) && v < T(1); }  // VOFutil.jl:144

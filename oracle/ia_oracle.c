@@ -1123,6 +1123,65 @@ static inline T viscF(const oc_ctx *c, int i, int jj, size_t I, const T *u, cons
     return mu * ((VC(u, i)[I] - VC(u, i)[I - ST(jj)]) + (VC(u, jj)[I] - VC(u, jj)[I - ST(i)]));
 }
 /* ------------------------------------------------------------------ */
+/* metrics.jl:15-50 (per-cell diagnostics; the Float64 literals make    */
+/* them Float64 for either element type) and their domain integrals     */
+/* ------------------------------------------------------------------ */
+/* ρkeI(I,u,f,λρ,U) = 0.25 Σ_i ((u[I,i]-U_i)² + (u[I+δi,i]-U_i)²) ρ(I) */
+static double rho_ke(const grid_t *g, size_t I, const T *u, const T *f, T lr, const double *U) {
+    double s = 0, rho = (double)lin(f[I], lr);
+    for (int i = 1; i <= g->D; ++i) {
+        double a = (double)VC(u, i)[I] - U[i - 1], b = (double)VC(u, i)[I + ST(i)] - U[i - 1];
+        s += (a * a + b * b) * rho;
+    }
+    return 0.25 * s;
+}
+/* ρuI(i,I,u,f,λρ,U) = 0.5 (u[I,i] + u[I+δi,i] - 2U_i) ρ(I) */
+static double rho_u(const grid_t *g, int i, size_t I, const T *u, const T *f, T lr, const double *U) {
+    return 0.5 * (((double)VC(u, i)[I] + (double)VC(u, i)[I + ST(i)]) - 2 * U[i - 1]) * (double)lin(f[I], lr);
+}
+/* ρgh(I,g,f,λρ,StatWL) = -ρ(I) Σ_i g_i (loc(0,I)_i - StatWL_i), loc(0,I) = I - 1.5 (1-based) */
+static double rho_gh(const grid_t *g, const int q[3], size_t I, const double *gv, const T *f, T lr, const double *wl) {
+    double s = 0;
+    for (int i = 0; i < g->D; ++i) s += gv[i] * ((q[i] - 1.5) - wl[i]);
+    return -(double)lin(f[I], lr) * s;
+}
+/* EnsI(I,ω) = ½ · ¼ · Σ over the 4 edges of cell I of ω² (per component in 3D,
+ * the edges parallel to axis i, ix = shiftDir(i,3,1), iy = shiftDir(i,3,2)) */
+static double ens(const grid_t *g, size_t I, const T *w) {
+    if (g->D == 2)
+        return 0.5 * 0.25 * (((double)w[I] * w[I] + (double)w[I + ST(1)] * w[I + ST(1)]) +
+                             (double)w[I + ST(2)] * w[I + ST(2)] + (double)w[I + ST(1) + ST(2)] * w[I + ST(1) + ST(2)]);
+    double s = 0;
+    for (int i = 1; i <= 3; ++i) {
+        int ix = i % 3 + 1, iy = (i + 1) % 3 + 1;
+        const T *wi = VC(w, i);
+        double a = wi[I], b = wi[I + ST(ix)], cc = wi[I + ST(iy)], d = wi[I + ST(ix) + ST(iy)];
+        s += ((a * a + b * b) + cc * cc) + d * d;
+    }
+    return 0.5 * 0.25 * s;
+}
+/* the vorticity the integral uses: ω_i = ∂_j u_k - ∂_k u_j with (i, j, k) cyclic
+ * (2D: ω = ∂_x v - ∂_y u) by backward differences, i.e. at the edge of cell I
+ * on its lower j and k sides -- the edges EnsI averages */
+static void vorticity(const grid_t *g, const T *u, T *w) {
+    rng_t R = r_all(g);
+    FOR_R(R) {
+        size_t I = IX(i, j, k);
+        int q[3] = {i, j, k};
+        if (g->D == 2) {
+            if (q[0] < 2 || q[1] < 2) { w[I] = 0; continue; }
+            w[I] = (VC(u, 2)[I] - VC(u, 2)[I - ST(1)]) - (VC(u, 1)[I] - VC(u, 1)[I - ST(2)]);
+            continue;
+        }
+        for (int a = 1; a <= 3; ++a) {
+            int jj = a % 3 + 1, kk = (a + 1) % 3 + 1;
+            if (q[jj - 1] < 2 || q[kk - 1] < 2) { VC(w, a)[I] = 0; continue; }
+            VC(w, a)[I] = (VC(u, kk)[I] - VC(u, kk)[I - ST(jj)]) - (VC(u, jj)[I] - VC(u, jj)[I - ST(kk)]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* surfaceTension.jl                                                    */
 /* ------------------------------------------------------------------ */
 static inline int contain_interface(T v) { return v > 0 && v < 1; } /* VOFutil.jl:144 */
@@ -1849,6 +1908,53 @@ real_t oc_f2alpha3(real_t m1, real_t m2, real_t m3, real_t v) { return f2a3(m1, 
 real_t oc_alpha2f3(real_t m1, real_t m2, real_t m3, real_t a) { return a2f3(m1, m2, m3, a); }
 real_t oc_koren(real_t u, real_t c, real_t d) { return limiter(0, u, c, d); }
 real_t oc_limiter(int lim, real_t u, real_t c, real_t d) { return limiter(lim, u, c, d); }
+double oc_rhoke(int D, const int *Ng, const real_t *u, const real_t *f, real_t lr, const int *I, const double *U) {
+    grid_t g;
+    grid_init(&g, D, Ng, 0);
+    const grid_t *gp = &g;
+    return rho_ke(gp, IXg(gp, I[0], I[1], D > 2 ? I[2] : 1), u, f, lr, U);
+}
+double oc_rhou(int D, const int *Ng, int i, const real_t *u, const real_t *f, real_t lr, const int *I, const double *U) {
+    grid_t g;
+    grid_init(&g, D, Ng, 0);
+    const grid_t *gp = &g;
+    return rho_u(gp, i, IXg(gp, I[0], I[1], D > 2 ? I[2] : 1), u, f, lr, U);
+}
+double oc_rhogh(int D, const int *Ng, const double *gv, const real_t *f, real_t lr, const int *I, const double *wl) {
+    grid_t g;
+    grid_init(&g, D, Ng, 0);
+    const grid_t *gp = &g;
+    int q[3] = {I[0], I[1], D > 2 ? I[2] : 1};
+    return rho_gh(gp, q, IXg(gp, q[0], q[1], q[2]), gv, f, lr, wl);
+}
+double oc_ens(int D, const int *Ng, const real_t *w, const int *I) {
+    grid_t g;
+    grid_init(&g, D, Ng, 0);
+    const grid_t *gp = &g;
+    return ens(gp, IXg(gp, I[0], I[1], D > 2 ? I[2] : 1), w);
+}
+/* domain integrals over the interior: out = {Σρke, Σρu_1..3, Σρgh, Σens, Σf} */
+int oc_metrics(oc_ctx *c, const double *U, const double *wl, double *out) {
+    grid_t *g = &c->g;
+    T *w = (T *)calloc(g->ns * (g->D == 3 ? 3 : 1), sizeof(T));
+    vorticity(g, c->u, w);
+    double gv[3] = {0, 0, 0};
+    if (c->has_g)
+        for (int d = 0; d < g->D; ++d) gv[d] = (double)c->gv[d];
+    for (int q = 0; q < 7; ++q) out[q] = 0;
+    rng_t R = r_in(g);
+    FOR_R(R) {
+        size_t I = IX(i, j, k);
+        int q[3] = {i, j, k};
+        out[0] += rho_ke(g, I, c->u, c->f, c->lr, U);
+        for (int a = 1; a <= g->D; ++a) out[a] += rho_u(g, a, I, c->u, c->f, c->lr, U);
+        out[4] += rho_gh(g, q, I, gv, c->f, c->lr, wl);
+        out[5] += ens(g, I, w);
+        out[6] += (double)c->f[I];
+    }
+    free(w);
+    return OC_OK;
+}
 real_t oc_popinet_height(int D, const int *Ng, const real_t *f, const int *I, int i) {
     grid_t g;
     grid_init(&g, D, Ng, 0);

@@ -114,6 +114,13 @@ real_t oc_limiter(int lim, real_t u, real_t c, real_t d);
 /* surfaceTension.jl: height (getPopinetHeight) and curvature (getCurvature) about
  * cell I (1-based) of a standalone block, along the signed axis i (1-based) */
 real_t oc_popinet_height(int D, const int *Ng, const real_t *f, const int *I, int i);
+/* metrics.jl:15-50 per cell (I 1-based, U / StatWL / g as doubles) and the domain integrals
+ * out = {Σρke, Σρu_1, Σρu_2, Σρu_3, Σρgh, Σenstrophy, Σf} over the interior */
+double oc_rhoke(int D, const int *Ng, const real_t *u, const real_t *f, real_t lr, const int *I, const double *U);
+double oc_rhou(int D, const int *Ng, int i, const real_t *u, const real_t *f, real_t lr, const int *I, const double *U);
+double oc_rhogh(int D, const int *Ng, const double *gv, const real_t *f, real_t lr, const int *I, const double *wl);
+double oc_ens(int D, const int *Ng, const real_t *w, const int *I);
+int oc_metrics(oc_ctx *c, const double *U, const double *wl, double *out);
 real_t oc_curvature(int D, const int *Ng, const real_t *f, const int *I, int i);
 void oc_set_threads(int n);
 const char *oc_last_error(oc_ctx *c);

@@ -82,6 +82,11 @@ def lib(T):
         "oc_koren": (R, [R, R, R]),
         "oc_limiter": (R, [C.c_int, R, R, R]),
         "oc_popinet_height": (R, [C.c_int, ip, vp, ip, C.c_int]),
+        "oc_rhoke": (C.c_double, [C.c_int, ip, vp, vp, R, ip, vp]),
+        "oc_rhou": (C.c_double, [C.c_int, ip, C.c_int, vp, vp, R, ip, vp]),
+        "oc_rhogh": (C.c_double, [C.c_int, ip, vp, vp, R, ip, vp]),
+        "oc_ens": (C.c_double, [C.c_int, ip, vp, ip]),
+        "oc_metrics": (C.c_int, [vp, vp, vp, vp]),
         "oc_curvature": (R, [C.c_int, ip, vp, ip, C.c_int]),
         "oc_normal_block": (C.c_int, [C.c_int, ip, vp, C.c_int, ip, vp]),
         "oc_getmu": (R, [C.c_int, ip, vp, C.c_int, C.c_int, ip, R, R, R]),
@@ -245,6 +250,17 @@ class OracleSim:
         self.lib.oc_project_fcg(self.ctx, float(dt_w), float(rtol), int(max_it), int(smooth_it), C.byref(it),
                                 C.byref(rel))
         return it.value, rel.value
+
+    def metrics(self, U=None, StatWL=None):
+        u = np.zeros(3)
+        w = np.zeros(3)
+        if U is not None:
+            u[:len(U)] = U
+        if StatWL is not None:
+            w[:len(StatWL)] = StatWL
+        out = np.zeros(7)
+        self.lib.oc_metrics(self.ctx, _ptr(u), _ptr(w), _ptr(out))
+        return out
 
     def cfl(self):
         return self.lib.oc_cfl(self.ctx)

@@ -521,3 +521,24 @@ def test_droplet_steps_with_surface_tension(D):
         assert g.flow.Δt[-1] == o.Δt[-1]
     assert np.abs(g.flow.u).max() > 0  # the surface tension drives a flow
     g.close()
+
+
+# --------------------------------------------------------------------------- diagnostics
+@pytest.mark.parametrize("D", [2, 3])
+def test_metrics_integrals(D):
+    """metrics.jl:15-50 integrated over the interior, GPU against the oracle."""
+    say("metrics integrals", f"{D}D")
+    case = bubble2d(32, np.float64) if D == 2 else bubbles3d(16, np.float64)
+    g, o = pair(case)
+    u = _swirl(case, np.float64, amp=0.3)
+    g.flow.u = u
+    o.set(O.U, u)
+    for U, wl in ((None, None), ((0.1, -0.2, 0.3)[:D], (0.0, 8.0, 4.0)[:D])):
+        mg = g.metrics(U, wl)
+        mo = o.metrics(U, wl)
+        got = [mg["kinetic_energy"], *mg["momentum"], *([0.0] * (3 - D)), mg["potential_energy"], mg["enstrophy"],
+               mg["volume"]]
+        for a, b in zip(got, mo):
+            assert abs(a - b) <= 1e-10 * max(1.0, abs(b)), (got, mo)
+    assert mg["kinetic_energy"] > 0 and mg["enstrophy"] > 0
+    g.close()

@@ -293,3 +293,29 @@ def test_surface_tension_kats():
     H = [L.oc_popinet_height(2, Ng, O._ptr(f), O._ints([x, 5]), 2) for x in (1, 2, 3)]
     assert H == [-0.3, -0.9, -1.4]
     assert abs(L.oc_curvature(2, Ng, O._ptr(f), O._ints([2, 5]), 2) - 0.0672718547928328) < 1e-15
+
+
+def test_metrics_kats():
+    """maintests.jl:321-347 (ρkeI, ρuI, ρgh, EnsI in 2D and 3D)."""
+    L = O.lib(np.float64)
+    u = np.zeros((4, 4, 2), order="F")
+    u[1, 1, 0] = 1; u[2, 1, 0] = 2; u[1, 1, 1] = 3; u[1, 2, 1] = 4
+    f = np.zeros((4, 4), order="F"); f[1, 1] = 0.5
+    Ng, I = O._ints([4, 4]), O._ints([2, 2])
+    d = lambda *v: np.ascontiguousarray(v, dtype=np.float64)
+    U0, U1 = d(0, 0, 0), d(1, 1, 0)
+    assert abs(L.oc_rhoke(2, Ng, O._ptr(u), O._ptr(f), 0.2, I, O._ptr(U0)) - 4.5) < 1e-14
+    assert abs(L.oc_rhoke(2, Ng, O._ptr(u), O._ptr(f), 0.2, I, O._ptr(U1)) - 2.1) < 1e-14
+    assert abs(L.oc_rhou(2, Ng, 1, O._ptr(u), O._ptr(f), 0.2, I, O._ptr(U0)) - 0.9) < 1e-14
+    assert abs(L.oc_rhou(2, Ng, 2, O._ptr(u), O._ptr(f), 0.2, I, O._ptr(U0)) - 2.1) < 1e-14
+    assert abs(L.oc_rhou(2, Ng, 1, O._ptr(u), O._ptr(f), 0.2, I, O._ptr(U1)) - 0.3) < 1e-14
+    assert abs(L.oc_rhogh(2, Ng, O._ptr(d(0, -1, 0)), O._ptr(f), 0.2, I, O._ptr(d(0, 0, 0))) - 0.3) < 1e-14
+    w = np.zeros((4, 4), order="F"); w[1, 1] = 1; w[2, 1] = 2; w[1, 2] = 3; w[2, 2] = 4
+    assert abs(L.oc_ens(2, Ng, O._ptr(w), I) - 3.75) < 1e-14
+    w3 = np.zeros((4, 4, 4, 3), order="F")
+    for c, cells in enumerate([((1, 1, 1), (1, 2, 1), (1, 1, 2), (1, 2, 2)),
+                               ((1, 1, 1), (1, 1, 2), (2, 1, 1), (2, 1, 2)),
+                               ((1, 1, 1), (2, 1, 1), (1, 2, 1), (2, 2, 1))]):
+        for v, cell in enumerate(cells):
+            w3[cell + (c,)] = v + 1
+    assert abs(L.oc_ens(3, O._ints([4, 4, 4]), O._ptr(w3), O._ints([2, 2, 2])) - 11.25) < 1e-14
