@@ -56,6 +56,7 @@ BYTES_PER_CELL = {
     "pcg_last": 4 + 4 + 8 + 8 + 4,                   # x, ϵ, r64, q64 -> x
     "fcg_Ap": 4 + 8 + 12 + 16,                       # w, p, L -> p', q
     "fcg_update": 32 + 16,                           # p', q, x64, r64 -> x64, r64
+    "redist": (12 + 16 + 16) / 3,                    # ϕ, ϕini (+ ϕ⁰ in stages 2, 3) -> ϕ', per stage
 }
 # kernel classes timed with HIP events in the timed region; the one with the
 # largest total time is reported as the dominant kernel
@@ -140,6 +141,25 @@ def poisson_compare(ia, N, rtol, warm=8, steps=3, lr=1e-3):
     return out
 
 
+def redistance_rate(ia, sim, cells, d=2.0, dτ=0.5):
+    """LevelSet(sim) + redistaning!(ls; d, dτ) on the state left by the timed
+    steps: round(d/dτ) SSP-RK3 steps of three fused stage launches, timed with
+    HIP events around each launch (after one untimed step)."""
+    ls = ia.LevelSet(sim)
+    ia.redistaning_(ls, d=dτ, dτ=dτ)
+    ls = ia.LevelSet(sim)
+    sim._call("ia_profile_enable", ia.PROFILE["redist"], 1)
+    ia.redistaning_(ls, d=d, dτ=dτ)
+    sim._call("ia_synchronize")
+    nl, ms = ia.C.c_int(), ia.C.c_double()
+    sim._call("ia_profile_read", ia.PROFILE["redist"], ia.C.byref(nl), ia.C.byref(ms))
+    sim._call("ia_profile_enable", ia.PROFILE["redist"], 0)
+    avg = ms.value / nl.value
+    return dict(d=d, dτ=dτ, launches=nl.value, total_ms=round(ms.value, 3), avg_ms=round(avg, 4),
+                GBps=round(BYTES_PER_CELL["redist"] * cells / (avg * 1e-3) / 1e9, 1),
+                bytes_per_launch=int(BYTES_PER_CELL["redist"] * cells))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +241,7 @@ def main():
                      "bytes_per_launch": int(bpl), "avg_launch_ms": avg_ms, "launches": timed[dom]["launches"]},
         "kernels": timed,
     }
+    out["levelset_redistance"] = redistance_rate(ia, sim, cells)
     sim.close()
     if not args.no_compare:
         out["config"]["poisson_compare"] = poisson_compare(ia, N, args.rtol)

@@ -179,6 +179,18 @@ int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats);
  * be NULL (zero). */
 int ia_metrics(ia_ctx *c, const double *U, const double *statwl, double *out);
 
+/* LevelSet(sim)  redistaning.jl:14-25: ϕ = 2f - 1 on every entry of cVOF.f⁰
+ * (IA_FIELD_F0 then reads ϕ) and ϕini = ϕ in a scratch buffer of the context
+ * (the reference's cVOF.fᶠ), valid until f is next advected or re-applied. */
+int ia_levelset(ia_ctx *c);
+/* redistaning!(ls; d, dτ, perdir)  redistaning.jl:44-56 (computeL! :67-108):
+ * round(d/dτ) SSP-RK3 pseudo-time steps of ∂τϕ = ϕini (1 - |∇ϕ|) on ϕ (f⁰),
+ * BCf!(ϕ; perdir) after every stage.  perdir is a bit mask (bit a-1: axis a
+ * periodic), passed separately as in the reference.  cVOF.α and Flow.σ
+ * (the reference's ϕ⁰ and L) are left holding scratch.  Needs ia_levelset
+ * since the last advection and at least 2 interior cells per axis. */
+int ia_redistance(ia_ctx *c, double d, double dtau, int perdir);
+
 /* Learned Poisson correction (no counterpart in the reference).  A five-layer
  * 3x3x3 convolutional network (3 -> 16 -> 16 -> 16 -> 16 -> 2 channels, ReLU
  * between layers), evaluated with BF16 matrix-core products and FP32
@@ -227,7 +239,8 @@ enum ia_prof_class {
     IA_PROF_FCG_AP = 8,       /* FCG p = w + βp, q = A p (FP64) */
     IA_PROF_FCG_UPDATE = 9,   /* FCG x += αp, r -= αq (FP64) */
     IA_PROF_PCG_LAST = 10,    /* last finest-level pcg! update fused with the FCG dots */
-    IA_PROF_NCLASS = 11
+    IA_PROF_REDIST = 11,      /* one level-set reinitialisation stage (computeL! + update + BCf!) */
+    IA_PROF_NCLASS = 12
 };
 int ia_profile_enable(ia_ctx *c, int kernel, int on);
 int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms);

@@ -24,7 +24,7 @@ FIELD = dict(f=0, f0=1, alpha=2, nhat=3, cbar=5, rhou=6, rhouf=7, u=9, u0=10, fo
 _VECTOR = {"nhat", "rhou", "rhouf", "u", "u0", "force", "mu0"}
 POISSON = {"reference": 0, "refined": 1, "learned": 2, "plain": 3}
 PROFILE = {"vof": 0, "momentum": 1, "visc_update": 2, "pcg_Ae": 3, "pcg_update": 4, "increment": 5,
-           "load_inc": 6, "prolong_inc": 7, "fcg_Ap": 8, "fcg_update": 9, "pcg_last": 10}
+           "load_inc": 6, "prolong_inc": 7, "fcg_Ap": 8, "fcg_update": 9, "pcg_last": 10, "redist": 11}
 STATUS = {0: "OK", 1: "NaN", 2: "divergence explosion", 3: "fill warning", 4: "HIP error", 5: "bad argument"}
 
 
@@ -89,6 +89,8 @@ def lib():
         "ia_profile_read": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "ia_last_error": (C.c_char_p, [vp]),
         "ia_metrics": (C.c_int, [vp, vp, vp, vp]),
+        "ia_levelset": (C.c_int, [vp]),
+        "ia_redistance": (C.c_int, [vp, C.c_double, C.c_double, C.c_int]),
         "ia_learned_init": (C.c_int, [vp, C.c_uint]),
         "ia_learned_set_weights": (C.c_int, [vp, vp, sz]),
         "ia_learned_get_weights": (C.c_int, [vp, vp, sz]),
@@ -385,8 +387,8 @@ class TwoPhaseSimulation:
         return v.value
 
     def sim_time(self):
-        """WaterLily sim_time: Σ Δt[1:end-1] scaled by U/L is the caller's choice; raw time here."""
-        return float(sum(self.flow.Δt[:-1]))
+        """WaterLily sim_time: the dimensionless time Σ Δt[1:end-1] · U/L."""
+        return float(sum(float(x) for x in self.flow.Δt[:-1])) * self.U / self.L
 
     def metrics(self, U=None, StatWL=None):
         """Domain integrals of the diagnostics of metrics.jl:15-50 (FP64): kinetic
@@ -473,22 +475,25 @@ def MPFMomStep_(flow, pois, intf, body=None, *, δt=None, udf=None):
     return st
 
 
-def sim_step_(sim, t_end=None, *, remeasure=False, udf=None, verbose=False):
+def sim_step_(sim, t_end=None, *, remeasure=False, udf=None, verbose=False, max_steps=None):
     """sim_step!(sim; remeasure, udf) (InterfaceAdvection.jl:100-103); with t_end,
-    WaterLily's generic loop `sim_step!(sim, t_end)` over that method."""
+    WaterLily's generic loop `sim_step!(sim, t_end)` over that method: steps while
+    the dimensionless time sim_time(sim) is below t_end (and fewer than max_steps
+    were taken)."""
     if t_end is None:
         return MPFMomStep_(sim, sim, sim, None, udf=udf)
-    t = sim.sim_time()
-    while t < t_end:
+    if not sim.U > 0:
+        raise ValueError("sim_step!(sim, t_end) needs a velocity scale U > 0 (sim_time is t·U/L)")
+    n0 = len(sim.flow.Δt)
+    while sim.sim_time() < t_end and (max_steps is None or len(sim.flow.Δt) - n0 < max_steps):
         MPFMomStep_(sim, sim, sim, None, udf=udf)
-        t = sim.sim_time()
         if verbose:
             sim_info(sim)
 
 
 def sim_info(sim):
     """sim_info (InterfaceAdvection.jl:104-107)."""
-    print(f"    tU/L={sim.sim_time() * sim.U / sim.L:10.6f}, ΔtU/L={sim.flow.Δt[-1] * sim.U / sim.L:.10f}")
+    print(f"    tU/L={sim.sim_time():10.6f}, ΔtU/L={sim.flow.Δt[-1] * sim.U / sim.L:.10f}")
 
 
 def advect_(flow, intf=None, f=None, u1=None, u2=None, dt=None):
@@ -505,6 +510,49 @@ def advect_(flow, intf=None, f=None, u1=None, u2=None, dt=None):
     sim._call("ia_advect", float(dt), len(sim.flow.Δt), C.byref(st))
     sim.last_stats = st
     return st
+
+
+class LevelSet:
+    """LevelSet(sim) (redistaning.jl:8-26): the signed level set ϕ = 2f - 1, kept
+    in the storage of sim.intf.f⁰ (so `ls.ϕ` is `sim.intf.f0`), with ϕini = ϕ in a
+    scratch buffer of the simulation and ϕ⁰ (`ls.ϕ0`) / L in sim.intf.α / sim.flow.σ.  It
+    stays valid until f is next advected; build a new one after a step."""
+
+    def __init__(self, sim):
+        sim._call("ia_levelset")
+        self.sim = sim
+        self.perdir = sim.perdir
+
+    @property
+    def ϕ(self):
+        return self.sim.intf.f0
+
+    @ϕ.setter
+    def ϕ(self, a):
+        self.sim.intf.f0 = a
+
+    phi = ϕ
+
+    @property
+    def ϕ0(self):
+        return self.sim.intf.α
+
+    phi0 = ϕ0
+
+    @property
+    def L(self):
+        return self.sim.flow.σ
+
+
+def redistaning_(ls, d=5, dτ=0.5, perdir=()):
+    """redistaning!(ls; d=5, dτ=0.5, perdir=()) (redistaning.jl:44-56): march
+    ∂τϕ = ϕini (1 - |∇ϕ|) with SSP-RK3 steps of dτ for a pseudo-time d.  As in the
+    reference, perdir is its own argument (default non-periodic), not ls.perdir."""
+    mask = 0
+    for a in perdir:
+        mask |= 1 << (int(a) - 1)
+    ls.sim._call("ia_redistance", float(d), float(dτ), mask)
+    return ls
 
 
 def inside(a, D=None):

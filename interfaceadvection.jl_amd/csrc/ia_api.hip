@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 using namespace ia;
@@ -80,6 +81,8 @@ struct ia_ctx {
         double last_ratio = 0;
     } cnn;
     void *st_buf = nullptr, *st_fb = nullptr;  // surface tension forcing (vector) and face-averaged f
+    int ls_ini = -1;          // pool buffer holding ϕini of the LevelSet (valid until f is next advected)
+    bool ls_valid = false;
     const int *halt = nullptr;   // the flexible CG's stop flag while a pipelined solve runs
     FcgState *pin = nullptr;     // pinned host copies of the solver state (two slots)
     hipEvent_t it_ev[2] = {nullptr, nullptr};
@@ -949,6 +952,43 @@ template <class T> struct Impl {
         return IA_OK;
     }
 
+    // ----------------------------------------------------------------- level set
+    // LevelSet(sim) (redistaning.jl:14-25): ϕ = 2f - 1 in f⁰'s storage, ϕini in a
+    // free buffer of the f pool (the reference's cVOF.fᶠ scratch)
+    static int levelset(ia_ctx *c) {
+        int q = free_fbuf(c, c->fi, c->f0i, -1);
+        long ns = c->g.ns;
+        hipLaunchKernelGGL(k_ls_init<T>, dim3((unsigned)((ns + NT - 1) / NT)), dim3(NT), 0, c->s, ns,
+                           P<T>(c->fb[c->fi]), P<T>(c->fb[c->f0i]), P<T>(c->fb[q]));
+        LAUNCHCHK(c);
+        c->ls_ini = q;
+        c->ls_valid = true;
+        return IA_OK;
+    }
+    // redistaning!(ls; d, dτ, perdir) (redistaning.jl:44-56): round(d/dτ) SSP-RK3
+    // steps, each stage one fused launch (computeL!, the stage update, BCf!).
+    // The stage outputs ping-pong ϕⁿ (f⁰) -> ϕ¹ (σ) -> ϕ² (α) -> ϕⁿ⁺¹ (f⁰), ϕⁿ
+    // staying in f⁰ as ϕ⁰ until the third stage overwrites it in place.
+    static int redistance(ia_ctx *c, long itmx, double dtau, int perdir) {
+        Grid g = c->g;
+        for (int a = 0; a < 3; ++a) g.per[a] = a < g.D ? (perdir >> a) & 1 : 0;
+        T dt = (T)dtau;
+        T *phi = P<T>(c->fb[c->f0i]), *s1 = P<T>(c->sig), *s2 = P<T>(c->al);
+        const T *ini = P<T>(c->fb[c->ls_ini]);
+        auto k = g.D == 3 ? k_ls_stage<3, T> : k_ls_stage<2, T>;
+        const T *in[3] = {phi, s1, s2};
+        T *out[3] = {s1, s2, phi};
+        const T al[3] = {T(0), (T)(3.0 / 4), (T)(1.0 / 3)};
+        for (long it = 0; it < itmx; ++it)
+            for (int q = 0; q < 3; ++q) {
+                prof_begin(c, IA_PROF_REDIST);
+                hipLaunchKernelGGL(k, blocks(c->t), dim3(NT), 0, c->s, g, c->t, in[q], phi, ini, out[q], dt, al[q]);
+                prof_end(c, IA_PROF_REDIST);
+                LAUNCHCHK(c);
+            }
+        return IA_OK;
+    }
+
     // metrics.jl integrals (Σρke, Σρu_i, Σρgh, Σ enstrophy, Σf)
     static int metrics(ia_ctx *c, const double *U, const double *wl, double *out) {
         Grid &g = c->g;
@@ -1397,6 +1437,7 @@ int ia_mg_levels(ia_ctx *c, int l, int *Ng) {
     return c->nlev;
 }
 int ia_apply_vof(ia_ctx *c, const void *sdf, size_t nbytes) {
+    if (c) c->ls_valid = false;
     return dispatch(c, [&](auto impl) { return decltype(impl)::apply_vof(c, sdf, nbytes); });
 }
 int ia_set_initial_velocity(ia_ctx *c, const void *u, size_t nbytes) {
@@ -1407,10 +1448,12 @@ int ia_cfl(ia_ctx *c, double *dt) {
     return dispatch(c, [&](auto impl) { return decltype(impl)::cfl(c, dt); });
 }
 int ia_mom_step(ia_ctx *c, double dt, int step_index, ia_step_stats *stats) {
+    if (c) c->ls_valid = false;
     if (stats) memset(stats, 0, sizeof *stats);
     return dispatch(c, [&](auto impl) { return decltype(impl)::mom_step(c, dt, step_index, stats); });
 }
 int ia_advect(ia_ctx *c, double dt, int step_index, ia_step_stats *stats) {
+    if (c) c->ls_valid = false;
     if (stats) memset(stats, 0, sizeof *stats);
     return dispatch(c, [&](auto impl) { return decltype(impl)::advect(c, dt, step_index, stats); });
 }
@@ -1419,6 +1462,7 @@ int ia_reconstruct(ia_ctx *c) {
 }
 int ia_vof_sweep(ia_ctx *c, int d, double dt, ia_step_stats *stats) {
     if (!c || d < 1 || d > c->g.D) return IA_ERR_BADARG;
+    c->ls_valid = false;
     if (stats) memset(stats, 0, sizeof *stats);
     return dispatch(c, [&](auto impl) { return decltype(impl)::vof_sweep_stage(c, d - 1, dt, stats); });
 }
@@ -1446,6 +1490,25 @@ int ia_metrics(ia_ctx *c, const double *U, const double *statwl, double *out) {
     double z[3] = {0, 0, 0};
     (void)hipSetDevice(c->dev);
     return dispatch(c, [&](auto impl) { return decltype(impl)::metrics(c, U ? U : z, statwl ? statwl : z, out); });
+}
+int ia_levelset(ia_ctx *c) {
+    if (!c) return IA_ERR_BADARG;
+    (void)hipSetDevice(c->dev);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::levelset(c); });
+}
+int ia_redistance(ia_ctx *c, double d, double dtau, int perdir) {
+    if (!c) return IA_ERR_BADARG;
+    if (!c->ls_valid) return fail(c, IA_ERR_BADARG, "ia_redistance: no level set (call ia_levelset after the last advection)");
+    if (perdir < 0 || perdir >= (1 << c->g.D)) return fail(c, IA_ERR_BADARG, "ia_redistance: perdir mask %d", perdir);
+    double r = d / dtau;
+    if (!std::isfinite(r)) return fail(c, IA_ERR_BADARG, "ia_redistance: d/dτ = %g is not finite", r);
+    r = std::nearbyint(r);  // round(d/dτ), ties to even
+    if (r > 1e8) return fail(c, IA_ERR_BADARG, "ia_redistance: %g pseudo-time steps", r);
+    for (int a = 0; a < c->g.D; ++a)
+        if (c->g.Ng[a] < 4) return fail(c, IA_ERR_BADARG, "ia_redistance: axis %d has fewer than 2 interior cells", a + 1);
+    (void)hipSetDevice(c->dev);
+    long itmx = r > 0 ? (long)r : 0;
+    return dispatch(c, [&](auto impl) { return decltype(impl)::redistance(c, itmx, dtau, perdir); });
 }
 int ia_learned_init(ia_ctx *c, unsigned seed) {
     if (!c || c->P.dtype != IA_F32 || c->g.D != 3 || (c->g.Ng[0] - 2) % 16) return IA_ERR_BADARG;

@@ -827,4 +827,60 @@ __global__ void __launch_bounds__(NT) k_reconstruct(Grid g, Tiling t, const T *f
             }
 }
 
+// ---------------------------------------------------------------------------
+// level-set reinitialisation (redistaning.jl)
+// ---------------------------------------------------------------------------
+// LevelSet(sim) (redistaning.jl:14-25): ϕ = 2f - 1 on every entry, ghosts
+// included, and ϕini = ϕ
+template <class T> __global__ void __launch_bounds__(NT) k_ls_init(long ns, const T *f, T *phi, T *ini) {
+    long q = (long)blockIdx.x * NT + threadIdx.x;
+    if (q >= ns) return;
+    T v = 2 * f[q] - 1;
+    phi[q] = v;
+    ini[q] = v;
+}
+// redistaning.jl:110 minmod
+template <class T> __device__ inline T ls_minmod(T a, T b) { return fabs(a) <= fabs(b) ? a : b; }
+// redistaning.jl:122-143 𝛁ϕᵢ²: second-order ENO (minmod-limited) one-sided
+// derivatives of ϕ[I-2..I+2], the upwind one for the sign s squared, else 0
+template <class T> __device__ inline T ls_grad2(T a, T b, T c, T d, T e, T s) {
+    T dp = d - c, dm = c - b;
+    T d2p = (e + c) - 2 * d, d2o = (d + b) - 2 * c, d2m = (c + a) - 2 * b;
+    T dR = dp - ls_minmod(d2p, d2o) / 2;
+    T dL = dm + ls_minmod(d2o, d2m) / 2;
+    T wR = dR * s, wL = dL * s;
+    if (wR < 0 && (wR + wL) < 0) return dR * dR;
+    if (wL > 0 && (wR + wL) > 0) return dL * dL;
+    return T(0);
+}
+// One stage of redistaning! (redistaning.jl:28-31, :67-108, BCf! of
+// VOFutil.jl:64-75):  L = ϕini (1 - |∇ϕ|) with |∇ϕ|² summed axis by axis
+// (the first and last interior cells of an axis take the mirrored (Neumann)
+// or wrapped (periodic) point two cells out), then
+// out = α ϕ⁰ + (1-α)(ϕ + dτ L), written with its ghost images.  ϕ and out are
+// different buffers; ϕ⁰ is read at I only and may be `out`.
+template <int D, class T>
+__global__ void __launch_bounds__(NT) k_ls_stage(Grid g, Tiling t, const T *phi, const T *phi0, const T *ini, T *out,
+                                                 T dtau, T alpha) {
+    int p[3];
+    if (!thread_cell(g, t, p)) return;
+    long I = lin(g, p);
+    T si = ini[I];
+    T s = si > 0 ? T(1) : (si < 0 ? T(-1) : si);
+    T c = phi[I];
+    T acc = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        long st = g.st[i], w = (long)(g.Ng[i] - 4) * st;
+        int q = p[i];
+        T b = phi[I - st], d = phi[I + st];
+        T a = q == 1 ? (g.per[i] ? phi[I + w] : b) : phi[I - 2 * st];
+        T e = q == g.Ng[i] - 2 ? (g.per[i] ? phi[I - w] : d) : phi[I + 2 * st];
+        acc = acc + ls_grad2(a, b, c, d, e, s);
+    }
+    T L = si * (T(1) - sqrt(acc));
+    T v = alpha * phi0[I] + (T(1) - alpha) * (c + dtau * L);
+    write_images<T>(g, out, p, v, 0, 0, T(0));
+}
+
 }  // namespace ia

@@ -2014,3 +2014,110 @@ int oc_vof_flux_face(int D, const int *Ng, real_t *ff, const real_t *f, const re
     vof_flux_face(g, ff, f, alpha, nhat, dl, d, IX(IFace[0], IFace[1], D > 2 ? IFace[2] : 1), rhouf, lr);
     return OC_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* redistaning.jl: level-set reinitialisation (LevelSet, redistaning!) */
+/* pinned by maintests.jl:264-319 (ramp, diagonal plane, planar front)  */
+/* plus the closed forms of tests/test_oracle_pins.py                   */
+/* ------------------------------------------------------------------ */
+/* redistaning.jl:110 minmod */
+static inline T ls_minmod(T a, T b) { return fabs(a) <= fabs(b) ? a : b; }
+/* redistaning.jl:122-143: second-order ENO upwind contribution to |∇ϕ|² along one axis */
+static T ls_grad2(T a, T b, T c, T d, T e, T s) {
+    T dp = d - c, dm = c - b;
+    T d2p = (e + c) - 2 * d, d2o = (d + b) - 2 * c, d2m = (c + a) - 2 * b;
+    T dR = dp - ls_minmod(d2p, d2o) / 2;
+    T dL = dm + ls_minmod(d2o, d2m) / 2;
+    T wR = dR * s, wL = dL * s;
+    if (wR < 0 && (wR + wL) < 0) return dR * dR;
+    if (wL > 0 && (wR + wL) > 0) return dL * dL;
+    return 0;
+}
+/* redistaning.jl:67-108 computeL!: L = ϕini (1 - |∇ϕ|) on the interior; per axis the
+ * first and last interior cells take the Neumann (mirrored) or periodic (wrapped)
+ * building block, the others the centred 5-point stencil */
+static void ls_computeL(const grid_t *g, T *L, const T *phi, const T *ini) {
+    for (size_t q = 0; q < g->ns; ++q) L[q] = 0;
+    for (int ii = 1; ii <= g->D; ++ii) {
+        long s = ST(ii);
+        int N = NG(ii);
+        rng_t lo = r_slice(g, 2, ii, 2), hi = r_slice(g, N - 1, ii, 2), in = r_in(g);
+        in.lo[ii - 1] = 3;
+        in.hi[ii - 1] = N - 2;
+        FOR_R(lo) {
+            size_t I = IX(i, j, k);
+            T a = PER(ii) ? phi[I + (long)(N - 4) * s] : phi[I - s];
+            L[I] += ls_grad2(a, phi[I - s], phi[I], phi[I + s], phi[I + 2 * s], tsign(ini[I]));
+        }
+        PFOR_R(in) {
+            size_t I = IX(i, j, k);
+            L[I] += ls_grad2(phi[I - 2 * s], phi[I - s], phi[I], phi[I + s], phi[I + 2 * s], tsign(ini[I]));
+        }
+        FOR_R(hi) {
+            size_t I = IX(i, j, k);
+            T e = PER(ii) ? phi[I - (long)(N - 4) * s] : phi[I + s];
+            L[I] += ls_grad2(phi[I - 2 * s], phi[I - s], phi[I], phi[I + s], e, tsign(ini[I]));
+        }
+    }
+    rng_t in = r_in(g);
+    PFOR_R(in) {
+        size_t I = IX(i, j, k);
+        L[I] = ini[I] * (1 - sqrt(L[I]));
+    }
+}
+/* redistaning.jl:28-31 _redistaningStage!: ϕ = α ϕ⁰ + (1-α)(ϕ + dτ L(ϕ)) on the interior */
+static void ls_stage(const grid_t *g, T *phi, const T *phi0, const T *ini, T *L, T dtau, T alpha) {
+    ls_computeL(g, L, phi, ini);
+    rng_t in = r_in(g);
+    PFOR_R(in) {
+        size_t I = IX(i, j, k);
+        phi[I] = alpha * phi0[I] + (1 - alpha) * (phi[I] + dtau * L[I]);
+    }
+}
+/* Julia round(x): nearest integer, ties to even */
+static double round_even(double x) {
+    double r = floor(x), f = x - r;
+    if (f > 0.5 || (f == 0.5 && fmod(r, 2.0) != 0)) r += 1;
+    return r;
+}
+/* redistaning.jl:44-56 redistaning!: round(d/dτ) SSP-RK3 steps, BCf! after each stage */
+static void ls_redistance(const grid_t *g, T *phi, T *phi0, const T *ini, T *L, double d, double dtau) {
+    long itmx = (long)round_even(d / dtau);
+    T dt = (T)dtau;
+    for (long it = 0; it < itmx; ++it) {
+        memcpy(phi0, phi, g->ns * sizeof(T));
+        ls_stage(g, phi, phi0, ini, L, dt, (T)0);
+        bc_f(g, phi, 0);
+        ls_stage(g, phi, phi0, ini, L, dt, (T)(3.0 / 4));
+        bc_f(g, phi, 0);
+        ls_stage(g, phi, phi0, ini, L, dt, (T)(1.0 / 3));
+        bc_f(g, phi, 0);
+    }
+}
+real_t oc_ls_grad2(real_t a, real_t b, real_t c, real_t d, real_t e, real_t s) { return ls_grad2(a, b, c, d, e, s); }
+int oc_ls_computeL(int D, const int *Ng, real_t *L, const real_t *phi, const real_t *ini, int perdir) {
+    grid_t g;
+    grid_init(&g, D, Ng, perdir);
+    for (int d = 0; d < D; ++d)
+        if (Ng[d] < 4) return OC_BADARG;
+    ls_computeL(&g, L, phi, ini);
+    return OC_OK;
+}
+int oc_ls_stage(int D, const int *Ng, real_t *phi, const real_t *phi0, const real_t *ini, real_t *L, real_t dtau,
+                real_t alpha, int perdir) {
+    grid_t g;
+    grid_init(&g, D, Ng, perdir);
+    for (int d = 0; d < D; ++d)
+        if (Ng[d] < 4) return OC_BADARG;
+    ls_stage(&g, phi, phi0, ini, L, dtau, alpha);
+    return OC_OK;
+}
+int oc_redistance(int D, const int *Ng, real_t *phi, real_t *phi0, const real_t *ini, real_t *L, double d, double dtau,
+                  int perdir) {
+    grid_t g;
+    grid_init(&g, D, Ng, perdir);
+    for (int q = 0; q < D; ++q)
+        if (Ng[q] < 4) return OC_BADARG;
+    ls_redistance(&g, phi, phi0, ini, L, d, dtau);
+    return OC_OK;
+}

@@ -122,6 +122,15 @@ double oc_rhogh(int D, const int *Ng, const double *gv, const real_t *f, real_t 
 double oc_ens(int D, const int *Ng, const real_t *w, const int *I);
 int oc_metrics(oc_ctx *c, const double *U, const double *wl, double *out);
 real_t oc_curvature(int D, const int *Ng, const real_t *f, const int *I, int i);
+/* redistaning.jl on standalone arrays of Ng points per axis (at least 2 interior
+ * cells each): the ENO contribution 𝛁ϕᵢ², computeL!, _redistaningStage! (in place,
+ * L as scratch) and redistaning!(ϕ; d, dτ, perdir) with ϕ⁰ and L as scratch */
+real_t oc_ls_grad2(real_t a, real_t b, real_t c, real_t d, real_t e, real_t s);
+int oc_ls_computeL(int D, const int *Ng, real_t *L, const real_t *phi, const real_t *ini, int perdir);
+int oc_ls_stage(int D, const int *Ng, real_t *phi, const real_t *phi0, const real_t *ini, real_t *L, real_t dtau,
+                real_t alpha, int perdir);
+int oc_redistance(int D, const int *Ng, real_t *phi, real_t *phi0, const real_t *ini, real_t *L, double d, double dtau,
+                  int perdir);
 void oc_set_threads(int n);
 const char *oc_last_error(oc_ctx *c);
 

@@ -95,6 +95,10 @@ def lib(T):
         "oc_bc_vof": (C.c_int, [C.c_int, ip, vp, vp, vp, C.c_int]),
         "oc_f2face": (C.c_int, [C.c_int, ip, vp, vp, C.c_int]),
         "oc_vof_flux_face": (C.c_int, [C.c_int, ip, vp, vp, vp, vp, R, C.c_int, ip, vp, R]),
+        "oc_ls_grad2": (R, [R, R, R, R, R, R]),
+        "oc_ls_computeL": (C.c_int, [C.c_int, ip, vp, vp, vp, C.c_int]),
+        "oc_ls_stage": (C.c_int, [C.c_int, ip, vp, vp, vp, vp, R, R, C.c_int]),
+        "oc_redistance": (C.c_int, [C.c_int, ip, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int]),
         "oc_set_threads": (None, [C.c_int]),
         "oc_last_iters": (C.c_int, [vp, C.c_int]),
         "oc_last_error": (C.c_char_p, [vp]),
@@ -120,6 +124,46 @@ def perdir_mask(perdir):
     for d in perdir:
         m |= 1 << (d - 1)
     return m
+
+
+def _ls_arrays(*arrs):
+    T = arrs[0].dtype
+    for a in arrs:
+        if a.dtype != T or not a.flags.f_contiguous:
+            raise ValueError("level-set arrays must share one float dtype and be Fortran-ordered")
+    return lib(T), T
+
+
+def ls_grad2(a, b, c, d, e, s, T=np.float64):
+    """𝛁ϕᵢ²(a,b,c,d,e,s) (redistaning.jl:122-143)."""
+    return lib(T).oc_ls_grad2(a, b, c, d, e, s)
+
+
+def computeL(L, phi, phiini, perdir=()):
+    """computeL!(L, ϕ, ϕini; perdir) (redistaning.jl:67-85), in place on L."""
+    lb, _ = _ls_arrays(L, phi, phiini)
+    st = lb.oc_ls_computeL(phi.ndim, _ints(phi.shape), _ptr(L), _ptr(phi), _ptr(phiini), perdir_mask(perdir))
+    if st:
+        raise ValueError("computeL!: every axis needs at least 2 interior cells")
+
+
+def redistaning_stage(phi, phi0, phiini, L, dτ, α, perdir=()):
+    """_redistaningStage!(ϕ, ϕ⁰, ϕini, L, dτ, α; perdir) (redistaning.jl:28-31), in place."""
+    lb, _ = _ls_arrays(phi, phi0, phiini, L)
+    st = lb.oc_ls_stage(phi.ndim, _ints(phi.shape), _ptr(phi), _ptr(phi0), _ptr(phiini), _ptr(L), dτ, α,
+                        perdir_mask(perdir))
+    if st:
+        raise ValueError("_redistaningStage!: every axis needs at least 2 interior cells")
+
+
+def redistaning(phi, phi0, phiini, L, d=5, dτ=0.5, perdir=()):
+    """redistaning!(ls; d, dτ, perdir) (redistaning.jl:44-56) on the arrays of a
+    LevelSet (ϕ updated in place; ϕ⁰ and L are scratch)."""
+    lb, _ = _ls_arrays(phi, phi0, phiini, L)
+    st = lb.oc_redistance(phi.ndim, _ints(phi.shape), _ptr(phi), _ptr(phi0), _ptr(phiini), _ptr(L), float(d),
+                          float(dτ), perdir_mask(perdir))
+    if st:
+        raise ValueError("redistaning!: every axis needs at least 2 interior cells")
 
 
 def sdf_samples(dims, sdf, T):
