@@ -65,6 +65,10 @@ struct ia_ctx {
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
     int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
+    // pcg! iterations on each coarse level inside Vcycle!: WaterLily's 6 (REFERENCE,
+    // PLAIN), and fcg_coarse_its(level cells) in the flexible CG's preconditioner
+    // (IA_FCG_CIT=a,b,... overrides levels 1, 2, ...; the last value repeats)
+    int cit_wl[MAXLEV], cit_fcg[MAXLEV];
     struct Cnn {
         bool ready = false;
         void *act[CNN_L + 1] = {};   // layer inputs/outputs, BF16 channels-last
@@ -335,8 +339,8 @@ template <class T> struct Impl {
         HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
         HIPCHK(c, hipMalloc(&c->fpart, (maxblk + FILL_R2) * sizeof(FillPart)));
         HIPCHK(c, hipMalloc(&c->st, sizeof(StepState)));
-        HIPCHK(c, hipMalloc(&c->pst, 64 * sizeof(PcgState)));
-        HIPCHK(c, hipMemsetAsync(c->pst, 0, 64 * sizeof(PcgState), c->s));
+        HIPCHK(c, hipMalloc(&c->pst, PCG_SLOTS * sizeof(PcgState)));
+        HIPCHK(c, hipMemsetAsync(c->pst, 0, PCG_SLOTS * sizeof(PcgState), c->s));
         HIPCHK(c, hipMalloc(&c->fst, sizeof(FcgState)));
         HIPCHK(c, hipMemsetAsync(c->fst, 0, sizeof(FcgState), c->s));
         HIPCHK(c, hipHostMalloc((void **)&c->pin, 2 * sizeof(FcgState)));
@@ -439,7 +443,7 @@ template <class T> struct Impl {
     // pre_done: Jacobi! + increment! of level l were fused into the caller's
     // residual load.  then_pcg: a pcg! on level l follows, so its first
     // statement is fused into the final prolongate! + increment!.
-    static void vcycle(ia_ctx *c, int l, bool pre_done = false, bool then_pcg = false) {
+    static void vcycle(ia_ctx *c, int l, const int *cit, bool pre_done = false, bool then_pcg = false) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
         if (!pre_done) jacobi(c, fl);
         hipLaunchKernelGGL(k_mg_restrict<T>, cb(co), dim3(NT), 0, c->s, view(co), fl.g, P<T>(fl.r));
@@ -459,12 +463,13 @@ template <class T> struct Impl {
             }
             a.l0 = l + 1;
             a.nlev = c->nlev;
+            for (int q = 0; q < MAXLEV; ++q) a.cit[q] = cit[q];
             a.halt = cur_halt;
             hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), c->tail_lds, c->s, a);
         } else {
             bool rec = l + 2 < c->nlev;
-            if (rec) vcycle(c, l + 1, false, true);
-            pcg(c, co, 6, rec);
+            if (rec) vcycle(c, l + 1, cit, false, true);
+            pcg(c, co, cit[l + 1], rec);
         }
         if (then_pcg) {
             bool f0 = l == 0;
@@ -502,7 +507,7 @@ template <class T> struct Impl {
         r2 = (double)(T)r2;
         int n = 0;
         while (r2 > tol && n < itmx) {
-            vcycle(c, 0, false, true);
+            vcycle(c, 0, c->cit_wl, false, true);
             pcg(c, l0, 6, true);
             hipLaunchKernelGGL(k_mg_center_l2<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR), ncell, 0,
                                pp(c, PP_ML2));
@@ -576,10 +581,10 @@ template <class T> struct Impl {
         hipLaunchKernelGGL(k_fcg_load_inc<T>, B, dim3(NT), 0, c->s, f, view(l0), pp(c, rbuf));
         prof_end(c, IA_PROF_LOAD_INC);
         if (c->smooth_it > 0) {
-            vcycle(c, 0, true, true);
+            vcycle(c, 0, c->cit_fcg, true, true);
             pcg(c, l0, c->smooth_it, true, &f, first);
         } else {
-            vcycle(c, 0, true, false);
+            vcycle(c, 0, c->cit_fcg, true, false);
             hipLaunchKernelGGL(k_fcg_dots<T>, B, dim3(NT), 0, c->s, f, pp(c, PP_FD), first);
         }
         prof_begin(c, IA_PROF_FCG_AP);
@@ -642,7 +647,7 @@ template <class T> struct Impl {
             prof_begin(c, IA_PROF_LOAD_INC);
             hipLaunchKernelGGL(k_fcg_load_inc<T>, B, dim3(NT), 0, c->s, f, view(l0), pp(c, rbuf));
             prof_end(c, IA_PROF_LOAD_INC);
-            vcycle(c, 0, true, true);
+            vcycle(c, 0, c->cit_wl, true, true);
             pcg(c, l0, 6, true);
             hipLaunchKernelGGL(k_ir_accum<T>, B, dim3(NT), 0, c->s, f);
             LAUNCHCHK(c);
@@ -1338,6 +1343,31 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     c->dev = p->device;
     c->es = p->dtype == IA_F64 ? 8 : 4;
     if (const char *gv = getenv("IA_USE_GRAPHS")) c->use_graphs = atoi(gv) != 0;
+    for (int l = 0; l < MAXLEV; ++l) {
+        long n = 1;
+        for (int a = 0; a < p->D; ++a) n *= std::max(1L, (long)(p->N[a] >> l));
+        c->cit_wl[l] = 6;
+        c->cit_fcg[l] = fcg_coarse_its(n);
+    }
+    if (const char *cv = getenv("IA_FCG_CIT")) {
+        int l = 1, v = 0;
+        const char *q = cv;
+        while (*q && l < MAXLEV) {
+            char *end;
+            v = (int)strtol(q, &end, 10);
+            if (end == q || v < 1 || v > CIT_MAX) {
+                delete c;
+                return IA_ERR_BADARG;
+            }
+            c->cit_fcg[l++] = v;
+            q = *end == ',' ? end + 1 : end;
+            if (end == q && *q) {
+                delete c;
+                return IA_ERR_BADARG;
+            }
+        }
+        for (; l < MAXLEV; ++l) c->cit_fcg[l] = v;
+    }
     int Ng[3];
     for (int a = 0; a < 3; ++a) Ng[a] = a < p->D ? p->N[a] + 2 : 1;
     grid_init(c->g, p->D, Ng, p->perdir);

@@ -1017,6 +1017,17 @@ __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, doub
 namespace ia {
 constexpr int NT1 = 1024;
 constexpr int MAXLEV = 12;
+// pcg! iterations per coarse level: at most CIT_MAX, three state slots each
+constexpr int CIT_MAX = 128;
+constexpr int PCG_SLOTS = 3 * CIT_MAX + 8;
+// the flexible CG preconditioner's pcg! iterations on a coarse level of n
+// interior cells.  Levels above 1M cells are bandwidth-bound and cost like
+// the fine level, so they keep WaterLily's 6; below, a pcg! iteration costs
+// little more than its launches, and solving the coarse problems more
+// exactly cuts the outer iterations (256^3 pool: 39 -> 12.6 per solve).
+__host__ __device__ inline int fcg_coarse_its(long n) {
+    return n > 1000000 ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
+}
 constexpr size_t TAIL_LDS_MAX = 152 * 1024;  // of the 160 KiB of LDS per CU
 template <class T> struct TailLev {
     Grid g;
@@ -1026,6 +1037,7 @@ template <class T> struct TailLev {
 };
 template <class T> struct TailArgs {
     TailLev<T> lv[MAXLEV];
+    int cit[MAXLEV];   // pcg! iterations per level
     int l0, nlev;
     const int *halt;
 };
@@ -1209,7 +1221,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
     // ascend: pcg!(l+1), prolongate!, increment!(l)
     for (int l = last - 1; l >= a.l0; --l) {
         SLev<T> f = slev(a, l, smem), cs = slev(a, l + 1, smem);
-        stail_pcg<D>(cs, 6, sh);
+        stail_pcg<D>(cs, a.cit[l + 1], sh);
         SLOOP(f, {
             int k0 = q[0] / 2, k1 = f.D > 1 ? q[1] / 2 : 0, k2 = f.D > 2 ? q[2] / 2 : 0;
             f.e[c] = cs.x[k0 + cs.n[0] * (k1 + cs.n[1] * k2)];
@@ -1217,7 +1229,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
         stail_increment<D>(f);
     }
     SLev<T> s = slev(a, a.l0, smem);
-    stail_pcg<D>(s, 6, sh);
+    stail_pcg<D>(s, a.cit[a.l0], sh);
     // the top level's solution back to global memory (interior)
     const TailLev<T> &G = a.lv[a.l0];
     SLOOP(s, {

@@ -591,6 +591,7 @@ struct oc_ctx {
     double mg_tol;
     int mg_itmx;
     int coarse_it; /* pcg! iterations on coarse levels inside Vcycle! (WaterLily: 6) */
+    int fcg_cit[16]; /* the same per level in the flexible CG's preconditioner (by level size) */
     int poisson_mode, smooth_it;
     double poisson_rtol;
     int last_iters[2];
@@ -730,16 +731,17 @@ static void mg_prolongate(lev_t *fl, const lev_t *c) {
         fl->eps[IX(i, j, k)] = c->x[IXg(gc, ci, cj, ck)];
     }
 }
-static void mg_vcycle(oc_ctx *c, int l) {
+static void mg_vcycle_it(oc_ctx *c, int l, const int *cit) {
     lev_t *fl = &c->lev[l], *co = &c->lev[l + 1];
     mg_jacobi(fl);
     mg_restrict(co, fl);
     for (size_t q = 0; q < co->g.ns; ++q) co->x[q] = 0;
-    if (l + 2 < c->nlev) mg_vcycle(c, l + 1);
-    mg_pcg(co, c->coarse_it);
+    if (l + 2 < c->nlev) mg_vcycle_it(c, l + 1, cit);
+    mg_pcg(co, cit ? cit[l + 1] : c->coarse_it);
     mg_prolongate(fl, co);
     mg_increment(fl);
 }
+static void mg_vcycle(oc_ctx *c, int l) { mg_vcycle_it(c, l, NULL); }
 static void mg_update(oc_ctx *c) {
     mg_set_diag(&c->lev[0]);
     for (int l = 1; l < c->nlev; ++l) {
@@ -1508,6 +1510,13 @@ oc_ctx *oc_create(const oc_params *p) {
     c->mg_tol = p->mg_tol > 0 ? p->mg_tol : 1e-4;
     c->mg_itmx = p->mg_itmx > 0 ? p->mg_itmx : 200;
     c->coarse_it = 6;
+    /* the flexible CG preconditioner's pcg! iterations on a coarse level of n
+     * interior cells (the HIP library's fcg_coarse_its) */
+    for (int l = 0; l < 16; ++l) {
+        long n = 1;
+        for (int a = 0; a < p->D; ++a) n *= (p->N[a] >> l) > 1 ? (p->N[a] >> l) : 1;
+        c->fcg_cit[l] = n > 1000000 ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
+    }
     c->poisson_mode = p->poisson_mode;
     c->poisson_rtol = p->poisson_rtol > 0 ? p->poisson_rtol : 1e-6;
     c->smooth_it = p->smooth_it;
@@ -1801,6 +1810,9 @@ static void mg_vcycle_sym(oc_ctx *c, int l, int npre, int npost, T omega, int nc
 int g_cycle_type = 0, g_npre = 2, g_npost = 2, g_ncoarse = 16;
 double g_omega = 0.8;
 void oc_set_coarse_it(oc_ctx *c, int it) { c->coarse_it = it; }
+void oc_set_fcg_cit(oc_ctx *c, const int *cit, int n) {
+    for (int l = 0; l < 16; ++l) c->fcg_cit[l] = l < n ? cit[l] : cit[n - 1];
+}
 void oc_set_cycle(int type, int npre, int npost, double omega, int ncoarse) {
     g_cycle_type = type; g_npre = npre; g_npost = npost; g_omega = omega; g_ncoarse = ncoarse;
 }
@@ -1815,7 +1827,7 @@ static void precond(oc_ctx *c, const double *r, double *w, double sc, int smooth
     if (g_cycle_type == 1) {
         mg_vcycle_sym(c, 0, g_npre, g_npost, (T)g_omega, g_ncoarse);
     } else {
-        mg_vcycle(c, 0);
+        mg_vcycle_it(c, 0, c->fcg_cit);
         if (smooth_it > 0) mg_pcg(l0, smooth_it);
     }
     per_bc(&c->g, l0->x);
