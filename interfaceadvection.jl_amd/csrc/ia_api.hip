@@ -29,6 +29,7 @@ struct Level {
     CTiling ct;
     void *L = nullptr, *Dg = nullptr, *iD = nullptr, *x = nullptr, *e = nullptr, *r = nullptr, *z = nullptr;
     void *e2 = nullptr, *z2 = nullptr;  // second ϵ/z pair of the fused pcg! iterations (levels >= 1)
+    void *r2 = nullptr, *w = nullptr, *w2 = nullptr;  // single-reduction CG: r, w pairs (levels >= 1)
     bool own = false;
 };
 
@@ -311,7 +312,7 @@ template <class T> struct Impl {
             lv.ct = ctiling(lv.g);
             size_t n = lv.g.ns;
             if (int r = alloc_zero(c, &lv.L, n * g.D * es)) return r;
-            void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z, &lv.e2, &lv.z2};
+            void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z, &lv.e2, &lv.z2, &lv.r2, &lv.w, &lv.w2};
             for (void **v : lb)
                 if (int r = alloc_zero(c, v, n * es)) return r;
             lv.own = true;
@@ -443,6 +444,21 @@ template <class T> struct Impl {
     // pre_done: Jacobi! + increment! of level l were fused into the caller's
     // residual load.  then_pcg: a pcg! on level l follows, so its first
     // statement is fused into the final prolongate! + increment!.
+    // the flexible CG preconditioner's coarse solve: single-reduction CG, one
+    // launch per iteration (k_cg1_init, k_cg1_step).  The final r may sit in
+    // r2; nothing reads a coarse r after its solve (restrict! rewrites it).
+    static void pcg1(ia_ctx *c, Level &l, int it) {
+        LevV<T> v = view(l);
+        dim3 B = cb(l);
+        T *r[2] = {P<T>(l.r), P<T>(l.r2)}, *w[2] = {P<T>(l.w), P<T>(l.w2)}, *s[2] = {P<T>(l.z), P<T>(l.z2)};
+        hipLaunchKernelGGL(k_cg1_init<T>, B, dim3(NT), 0, c->s, v, w[0], c->pst, pp(c, PP_A));
+        for (int q = 1; q <= it; ++q) {
+            int a = (q - 1) & 1;
+            Cg1V<T> b{r[a], w[a], s[a], r[1 - a], w[1 - a], s[1 - a]};
+            hipLaunchKernelGGL(k_cg1_step<T>, B, dim3(NT), 0, c->s, v, b, c->pst, q, it,
+                               pp(c, (q & 1) ? PP_A : PP_B), pp(c, (q & 1) ? PP_B : PP_A));
+        }
+    }
     static void vcycle(ia_ctx *c, int l, const int *cit, bool pre_done = false, bool then_pcg = false) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
         if (!pre_done) jacobi(c, fl);
@@ -469,7 +485,8 @@ template <class T> struct Impl {
         } else {
             bool rec = l + 2 < c->nlev;
             if (rec) vcycle(c, l + 1, cit, false, true);
-            pcg(c, co, cit[l + 1], rec);
+            if (cit == c->cit_fcg) pcg1(c, co, cit[l + 1]);
+            else pcg(c, co, cit[l + 1], rec);
         }
         if (then_pcg) {
             bool f0 = l == 0;
@@ -1349,7 +1366,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
         c->cit_wl[l] = 6;
         c->cit_fcg[l] = fcg_coarse_its(n);
     }
-    if (const char *cv = getenv("IA_FCG_CIT")) {
+    if (const char *cv = getenv("IA_FCG_CIT"); cv && *cv) {
         int l = 1, v = 0;
         const char *q = cv;
         while (*q && l < MAXLEV) {
@@ -1404,7 +1421,7 @@ void ia_destroy(ia_ctx *c) {
         for (void *b : lb)
             if (b) (void)hipFree(b);
         if (lv.own) {
-            void *ob[] = {lv.L, lv.x, lv.z, lv.e2, lv.z2};
+            void *ob[] = {lv.L, lv.x, lv.z, lv.e2, lv.z2, lv.r2, lv.w, lv.w2};
             for (void *b : ob)
                 if (b) (void)hipFree(b);
         }

@@ -642,6 +642,134 @@ __global__ void __launch_bounds__(NT) k_pcg_eps(LevV<T> l, PcgState *S, int sin,
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) S[sout] = st;
 }
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned CG in the single-reduction form of Chronopoulos and
+// Gear (1989): the flexible CG preconditioner's solver on coarse levels above
+// the one-workgroup tail (no reference counterpart; WaterLily's pcg! needs two
+// reductions, so two launches, per iteration).  With u = r·iD and w = A u:
+//   init:  u₀, w₀ = A u₀, partials γ₀ = r·u₀, δ₀ = u₀·w₀
+//   step:  β = γ/γ₋, α = γ/(δ − βγ/α₋)  (first step: α = γ/δ, p = u, s = w)
+//          p = u + βp, s = w + βs, x += αp, r' = r − αs, then
+//          w' = A u' with u'(K) = r'(K)·iD(K) formed on the fly at the cell and
+//          its neighbours, partials γ' = r'·u', δ' = u'·w'
+// so one launch per iteration.  r, w and s are read at neighbours, so their
+// new values go to the other buffer of a pair; p and x are updated in place.
+// Stops (flags in the PcgState slot, as pcg!) after `it` updates of x, after
+// an update with |α| < 1e-2, or when |γ| < 10eps.
+// ---------------------------------------------------------------------------
+template <class T> struct Cg1V {
+    const T *r, *w, *s;   // read (previous iterate), also at neighbours
+    T *ro, *wo, *so;      // written
+};
+template <class T>
+__global__ void __launch_bounds__(NT) k_cg1_init(LevV<T> l, T *w, PcgState *S, double *part) {
+    HALT_IF(l)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PcgState st;
+        st.rho = 0;
+        st.alpha = 0;
+        st.done = 0;
+        st.stop = 0;
+        S[0] = st;
+    }
+    const T *__restrict__ r = l.r;
+    const T *__restrict__ iD = l.iD;
+    T *__restrict__ pd = l.e;  // search direction
+    T *__restrict__ s = l.z;
+    double g = 0, d = 0;
+    DISPATCH_D(l.g, {
+        col_pipe(l.g, l.t,
+                 [&](const int *q, long I) { return ld7<D>(l.g, q, I, l.L, [&](long K) { return r[K] * iD[K]; }); },
+                 [&](const int *, long I, const S7<D, T> &st) {
+                     T ww = mul7<D>(st);
+                     pd[I] = st.c;
+                     s[I] = ww;
+                     w[I] = ww;
+                     g += (double)r[I] * (double)st.c;
+                     d += (double)st.c * (double)ww;
+                 });
+    })
+    block_partial(g, part, 2, 0);
+    block_partial(d, part, 2, 1);
+}
+template <class T>
+__global__ void __launch_bounds__(NT) k_cg1_step(LevV<T> l, Cg1V<T> b, PcgState *S, int q, int it,
+                                                 const double *pin, double *pout) {
+    HALT_IF(l)
+    PcgState st = S[q - 1];
+    T alpha = 0, beta = 0;
+    if (!st.done) {
+        if (st.stop) {
+            st.done = 1;
+        } else {
+            T gam = (T)block_total(pin, l.t.nblk, 2, 0);
+            T del = (T)block_total(pin, l.t.nblk, 2, 1);
+            if (fabs(gam) < 10 * eps_of<T>()) {
+                st.done = 1;
+            } else {
+                if (q == 1) {
+                    alpha = gam / del;
+                } else {
+                    beta = gam / (T)st.rho;
+                    alpha = gam / (del - (beta * gam) / (T)st.alpha);
+                }
+                st.rho = (double)gam;
+                st.alpha = (double)alpha;
+                st.stop = (q == it || fabs((double)alpha) < 1e-2) ? 1 : 0;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[q] = st;
+    if (st.done) return;
+    const T *__restrict__ r = b.r;
+    const T *__restrict__ w = b.w;
+    const T *__restrict__ s = b.s;
+    const T *__restrict__ iD = l.iD;
+    T *__restrict__ pd = l.e;  // search direction (COL_LOOP declares p[3], the cell)
+    T *__restrict__ x = l.x;
+    T *__restrict__ ro = b.ro;
+    T *__restrict__ so = b.so;
+    const bool first = q == 1;
+    // s and r' at any cell, from the previous iterate
+    auto sK = [&](long K) { return first ? w[K] : w[K] + beta * s[K]; };
+    auto rK = [&](long K) { return r[K] - alpha * sK(K); };
+    if (st.stop) {
+        // last update: x and r only
+        COL_LOOP(l, {
+            T u = r[I] * iD[I];
+            T pp = first ? u : u + beta * pd[I];
+            T ss = sK(I);
+            pd[I] = pp;
+            so[I] = ss;
+            x[I] = x[I] + alpha * pp;
+            ro[I] = r[I] - alpha * ss;
+        })
+        return;
+    }
+    T *__restrict__ wo = b.wo;
+    double g = 0, d = 0;
+    DISPATCH_D(l.g, {
+        col_pipe(l.g, l.t,
+                 [&](const int *qq, long I) { return ld7<D>(l.g, qq, I, l.L, [&](long K) { return rK(K) * iD[K]; }); },
+                 [&](const int *, long I, const S7<D, T> &sv) {
+                     T u = r[I] * iD[I];
+                     T pp = first ? u : u + beta * pd[I];
+                     T ss = sK(I);
+                     T rr = r[I] - alpha * ss;
+                     T ww = mul7<D>(sv);
+                     pd[I] = pp;
+                     so[I] = ss;
+                     x[I] = x[I] + alpha * pp;
+                     ro[I] = rr;
+                     wo[I] = ww;
+                     g += (double)rr * (double)sv.c;
+                     d += (double)sv.c * (double)ww;
+                 });
+    })
+    block_partial(g, pout, 2, 0);
+    block_partial(d, pout, 2, 1);
+}
+
 // perBC!(x): periodic images of the interior
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_perbc(LevV<T> l, T *x) {

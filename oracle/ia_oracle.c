@@ -592,6 +592,9 @@ struct oc_ctx {
     int mg_itmx;
     int coarse_it; /* pcg! iterations on coarse levels inside Vcycle! (WaterLily: 6) */
     int fcg_cit[16]; /* the same per level in the flexible CG's preconditioner (by level size) */
+    int fcg_cg1;     /* coarse levels above the tail use mg_pcg1 in the flexible CG's preconditioner */
+    int tail_level;  /* first level of at most 4096 cells (the HIP library's one-workgroup tail) */
+    T *cg1_u, *cg1_w, *cg1_s; /* mg_pcg1 scratch, level-1 sized */
     int poisson_mode, smooth_it;
     double poisson_rtol;
     int last_iters[2];
@@ -682,6 +685,51 @@ static void mg_pcg(lev_t *lv, int it) {
         rho = rho2;
     }
 }
+/* Jacobi-preconditioned CG in the single-reduction form of Chronopoulos & Gear
+ * (1989), the flexible CG preconditioner's smoother on coarse levels (no
+ * reference counterpart; WaterLily's pcg! needs two reductions per
+ * iteration).  With u = r·iD and w = A u, each step takes
+ * β = γ/γ₋, α = γ/(δ − βγ/α₋)  (α = γ/δ at the start), p = u + βp,
+ * s = w + βs, x += αp, r −= αs, then γ = r·u, δ = u·w of the new r.
+ * Stops after `it` updates of x, after an update with |α| < 1e-2, or when
+ * |γ| < 10eps (as pcg!).  `w`, `s` are scratch of the level's size. */
+static void mg_pcg1(lev_t *lv, int it, T *u, T *w, T *s) {
+    grid_t *g = &lv->g;
+    rng_t R = r_in(g);
+    T *x = lv->x, *r = lv->r, *p = lv->eps;
+    PFOR_R(R) { size_t I = IX(i, j, k); u[I] = r[I] * lv->iD[I]; }
+    per_bc(g, u);
+    PFOR_R(R) { size_t I = IX(i, j, k); w[I] = mg_mult(g, I, lv->L, lv->Dg, u); }
+    T gam = (T)r_dot(g, R, r, u), del = (T)r_dot(g, R, u, w);
+    T gprev = 0, aprev = 0;
+    for (int q = 1; q <= it; ++q) {
+        if (fabs(gam) < 10 * EPS_T) return;
+        T alpha, beta = 0;
+        if (q == 1) {
+            alpha = gam / del;
+        } else {
+            beta = gam / gprev;
+            alpha = gam / (del - (beta * gam) / aprev);
+        }
+        PFOR_R(R) {
+            size_t I = IX(i, j, k);
+            T pp = q == 1 ? u[I] : u[I] + beta * p[I];
+            T ss = q == 1 ? w[I] : w[I] + beta * s[I];
+            p[I] = pp;
+            s[I] = ss;
+            x[I] = x[I] + alpha * pp;
+            r[I] = r[I] - alpha * ss;
+        }
+        if (q == it || fabs((double)alpha) < 1e-2) return;
+        PFOR_R(R) { size_t I = IX(i, j, k); u[I] = r[I] * lv->iD[I]; }
+        per_bc(g, u);
+        PFOR_R(R) { size_t I = IX(i, j, k); w[I] = mg_mult(g, I, lv->L, lv->Dg, u); }
+        gprev = gam;
+        aprev = alpha;
+        gam = (T)r_dot(g, R, r, u);
+        del = (T)r_dot(g, R, u, w);
+    }
+}
 /* restrict!(a,b): a[I] = Σ_{J∈up(I)} b[J],  up(I) = 2I-2 : 2I-1 */
 static void mg_restrict(lev_t *c, const lev_t *fl) {
     const grid_t *g = &c->g, *gf = &fl->g;
@@ -737,7 +785,10 @@ static void mg_vcycle_it(oc_ctx *c, int l, const int *cit) {
     mg_restrict(co, fl);
     for (size_t q = 0; q < co->g.ns; ++q) co->x[q] = 0;
     if (l + 2 < c->nlev) mg_vcycle_it(c, l + 1, cit);
-    mg_pcg(co, cit ? cit[l + 1] : c->coarse_it);
+    if (cit && c->fcg_cg1 && l + 1 < c->tail_level)
+        mg_pcg1(co, cit[l + 1], c->cg1_u, c->cg1_w, c->cg1_s);
+    else
+        mg_pcg(co, cit ? cit[l + 1] : c->coarse_it);
     mg_prolongate(fl, co);
     mg_increment(fl);
 }
@@ -808,6 +859,17 @@ static int mg_build(oc_ctx *c) {
         snprintf(c->msg, sizeof c->msg, "MultiLevelPoisson requires size=a2^n, where n>2");
         return OC_BADARG;
     }
+    /* the first level of at most 4096 interior cells: where the HIP library's
+     * one-workgroup tail (which runs pcg!) begins */
+    c->tail_level = c->nlev;
+    for (int l = 1; l < c->nlev; ++l) {
+        long n = rng_count(r_in(&c->lev[l].g));
+        if (n <= 4096) { c->tail_level = l; break; }
+    }
+    size_t n1 = c->lev[1].g.ns;
+    c->cg1_u = alloc_fill(n1, 0);
+    c->cg1_w = alloc_fill(n1, 0);
+    c->cg1_s = alloc_fill(n1, 0);
     return OC_OK;
 }
 
@@ -1517,6 +1579,7 @@ oc_ctx *oc_create(const oc_params *p) {
         for (int a = 0; a < p->D; ++a) n *= (p->N[a] >> l) > 1 ? (p->N[a] >> l) : 1;
         c->fcg_cit[l] = n > 1000000 ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
     }
+    c->fcg_cg1 = 1;
     c->poisson_mode = p->poisson_mode;
     c->poisson_rtol = p->poisson_rtol > 0 ? p->poisson_rtol : 1e-6;
     c->smooth_it = p->smooth_it;
@@ -1551,6 +1614,7 @@ void oc_destroy(oc_ctx *c) {
     T *bufs[] = {c->u, c->u0, c->fv, c->p, c->sig, c->mu0, c->f, c->f0, c->al, c->nh, c->ff, c->ru, c->ruf, c->dr};
     for (unsigned q = 0; q < sizeof bufs / sizeof bufs[0]; ++q) free(bufs[q]);
     free(c->cb);
+    free(c->cg1_u); free(c->cg1_w); free(c->cg1_s);
     for (int l = 0; l < c->nlev; ++l) {
         lev_t *lv = &c->lev[l];
         free(lv->Dg); free(lv->iD); free(lv->eps); free(lv->r);
@@ -1810,6 +1874,7 @@ static void mg_vcycle_sym(oc_ctx *c, int l, int npre, int npost, T omega, int nc
 int g_cycle_type = 0, g_npre = 2, g_npost = 2, g_ncoarse = 16;
 double g_omega = 0.8;
 void oc_set_coarse_it(oc_ctx *c, int it) { c->coarse_it = it; }
+void oc_set_fcg_cg1(oc_ctx *c, int on) { c->fcg_cg1 = on; }
 void oc_set_fcg_cit(oc_ctx *c, const int *cit, int n) {
     for (int l = 0; l < 16; ++l) c->fcg_cit[l] = l < n ? cit[l] : cit[n - 1];
 }

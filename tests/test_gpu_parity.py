@@ -241,13 +241,15 @@ def test_project_reference(T, per):
     g.close()
 
 
-def test_project_refined_reaches_1e6():
-    say("FP64 flexible CG + FP32 V-cycle, 64^2")
-    case = bubble2d(64, np.float32)
+@pytest.mark.parametrize("N", [64, 256])
+def test_project_refined_reaches_1e6(N):
+    """At 256^2 the 128^2 level runs the single-reduction CG (above the tail)."""
+    say(f"FP64 flexible CG + FP32 V-cycle, {N}^2")
+    case = bubble2d(N, np.float32)
     g, o = pair(case, poisson="refined", poisson_rtol=1e-6, poisson_max_outer=100)
     rng = np.random.default_rng(8)
     u = np.zeros(g.Ng + (2,), dtype=np.float32, order="F")
-    u[1:-1, 1:-1] = rng.standard_normal((64, 64, 2))
+    u[1:-1, 1:-1] = rng.standard_normal((N, N, 2))
     g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
     o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
     g._call("ia_update_L", 0)
@@ -264,11 +266,12 @@ def test_project_refined_reaches_1e6():
     g.close()
 
 
-@pytest.mark.parametrize("per,N", [(False, 32), (True, 32), (False, 128)])
+@pytest.mark.parametrize("per,N", [(False, 32), (True, 32), (False, 128), (True, 128)])
 def test_project_refined_3d(per, N):
     """3D takes the LDS-tiled solver kernels; walls and fully periodic.  At 128^3
-    the preconditioner's coarse levels of 64^3 and 32^3 run 48 and 64 pcg!
-    iterations outside the one-workgroup tail (16^3 and below, 32 each)."""
+    the preconditioner's coarse levels of 64^3 and 32^3 run 48 and 64 iterations
+    of the single-reduction CG outside the one-workgroup tail (16^3 and below,
+    32 pcg! iterations each)."""
     say(f"FCG + V-cycle 3D {N}^3", "periodic" if per else "walls")
     case = bubbles3d(N, np.float32)
     if per:
