@@ -455,8 +455,13 @@ template <class T> struct Impl {
         for (int q = 1; q <= it; ++q) {
             int a = (q - 1) & 1;
             Cg1V<T> b{r[a], w[a], s[a], r[1 - a], w[1 - a], s[1 - a]};
-            hipLaunchKernelGGL(k_cg1_step<T>, B, dim3(NT), 0, c->s, v, b, c->pst, q, it,
-                               pp(c, (q & 1) ? PP_A : PP_B), pp(c, (q & 1) ? PP_B : PP_A));
+            double *pin = pp(c, (q & 1) ? PP_A : PP_B), *pout = pp(c, (q & 1) ? PP_B : PP_A);
+            if (l.ct.zb == 1) {
+                auto k = l.g.D == 3 ? k_cg1_step1<3, T> : k_cg1_step1<2, T>;
+                hipLaunchKernelGGL(k, B, dim3(NT), 0, c->s, v, b, c->pst, q, it, pin, pout);
+            } else {
+                hipLaunchKernelGGL(k_cg1_step<T>, B, dim3(NT), 0, c->s, v, b, c->pst, q, it, pin, pout);
+            }
         }
     }
     static void vcycle(ia_ctx *c, int l, const int *cit, bool pre_done = false, bool then_pcg = false) {

@@ -770,6 +770,162 @@ __global__ void __launch_bounds__(NT) k_cg1_step(LevV<T> l, Cg1V<T> b, PcgState 
     block_partial(d, pout, 2, 1);
 }
 
+// two block sums in one pass (same per-thread order and tree as two calls of
+// block_sum / block_total, so the same values)
+__device__ inline void block_sum2(double &a, double &b) {  // valid in thread 0
+    __shared__ double sh2[2][NT / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sh2[0][w] = a;
+        sh2[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0, sb = 0;
+        for (int q = 0; q < NT / 64; ++q) {
+            sa += sh2[0][q];
+            sb += sh2[1][q];
+        }
+        a = sa;
+        b = sb;
+    }
+}
+__device__ inline void block_total2(const double *part, long n, double &a, double &b) {
+    __shared__ double tot2[2];
+    double va = 0, vb = 0;
+    for (long q = threadIdx.x; q < n; q += NT) {
+        va += part[2 * q];
+        vb += part[2 * q + 1];
+    }
+    block_sum2(va, vb);
+    if (threadIdx.x == 0) {
+        tot2[0] = va;
+        tot2[1] = vb;
+    }
+    __syncthreads();
+    a = tot2[0];
+    b = tot2[1];
+    __syncthreads();
+}
+// k_cg1_step for tilings of one cell per thread (zb == 1, or 2D): every load
+// the thread needs -- the previous partials, and r, w, s, iD at the cell and
+// its neighbours -- is issued before the partial sums are reduced, so the
+// launch waits for one memory round trip instead of three.  Same arithmetic.
+template <int D, class T>
+__global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState *S, int q, int it,
+                                                  const double *pin, double *pout) {
+    HALT_IF(l)
+    const Grid &g = l.g;
+    int p[3], zlo, zhi;
+    bool act = thread_col(g, l.t, p, zlo, zhi);
+    p[2] = zlo;
+    long I = (long)p[0] + g.st[1] * p[1] + g.st[2] * zlo;
+    PcgState st = S[q - 1];
+    // the previous launch's partials (strided per thread, as block_total)
+    double va = 0, vb = 0;
+    for (long k = threadIdx.x; k < l.t.nblk; k += NT) {
+        va += pin[2 * k];
+        vb += pin[2 * k + 1];
+    }
+    // raw values at the cell and its neighbours
+    T rc = 0, wc = 0, sc = 0, dc = 0, pc = 0, xc = 0;
+    T rm[D], rp[D], wm[D], wp[D], sm[D], sp[D], dm[D], dp[D], lm[D], lp[D];
+    if (act) {
+        Nb n = neighbours_d<D>(g, p);
+        rc = b.r[I];
+        wc = b.w[I];
+        sc = b.s[I];
+        dc = l.iD[I];
+        pc = l.e[I];
+        xc = l.x[I];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            long Km = I + n.m[i], Kp = I + n.p[i];
+            rm[i] = b.r[Km];
+            rp[i] = b.r[Kp];
+            wm[i] = b.w[Km];
+            wp[i] = b.w[Kp];
+            sm[i] = b.s[Km];
+            sp[i] = b.s[Kp];
+            dm[i] = l.iD[Km];
+            dp[i] = l.iD[Kp];
+            lm[i] = l.L[g.ns * i + I];
+            lp[i] = l.L[g.ns * i + I + g.st[i]];
+        }
+    }
+    T alpha = 0, beta = 0;
+    bool reduce = !st.done && !st.stop;
+    if (reduce) {
+        // block_total2, with the per-thread sums already loaded
+        __shared__ double tot2[2];
+        block_sum2(va, vb);
+        if (threadIdx.x == 0) {
+            tot2[0] = va;
+            tot2[1] = vb;
+        }
+        __syncthreads();
+        T gam = (T)tot2[0];
+        T del = (T)tot2[1];
+        __syncthreads();
+        if (fabs(gam) < 10 * eps_of<T>()) {
+            st.done = 1;
+        } else {
+            if (q == 1) {
+                alpha = gam / del;
+            } else {
+                beta = gam / (T)st.rho;
+                alpha = gam / (del - (beta * gam) / (T)st.alpha);
+            }
+            st.rho = (double)gam;
+            st.alpha = (double)alpha;
+            st.stop = (q == it || fabs((double)alpha) < 1e-2) ? 1 : 0;
+        }
+    } else if (!st.done) {
+        st.done = 1;  // the previous launch made the last update
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[q] = st;
+    if (st.done) return;
+    const bool first = q == 1;
+    double gd = 0, dd = 0;
+    if (act) {
+        T u = rc * dc;
+        T pp = first ? u : u + beta * pc;
+        T ss = first ? wc : wc + beta * sc;
+        T rr = rc - alpha * ss;
+        l.e[I] = pp;
+        b.so[I] = ss;
+        l.x[I] = xc + alpha * pp;
+        b.ro[I] = rr;
+        if (!st.stop) {
+            // u' = r'·iD at the cell and its neighbours, w' = A u' (mul7's order)
+            S7<D, T> sv;
+            sv.c = rr * dc;
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                T smv = first ? wm[i] : wm[i] + beta * sm[i];
+                T spv = first ? wp[i] : wp[i] + beta * sp[i];
+                sv.m[i] = (rm[i] - alpha * smv) * dm[i];
+                sv.p[i] = (rp[i] - alpha * spv) * dp[i];
+                sv.lm[i] = lm[i];
+                sv.lp[i] = lp[i];
+            }
+            T ww = mul7<D>(sv);
+            b.wo[I] = ww;
+            gd = (double)rr * (double)sv.c;
+            dd = (double)sv.c * (double)ww;
+        }
+    }
+    if (st.stop) return;
+    block_sum2(gd, dd);
+    if (threadIdx.x == 0) {
+        pout[2 * blockIdx.x] = gd;
+        pout[2 * blockIdx.x + 1] = dd;
+    }
+}
+
 // perBC!(x): periodic images of the interior
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_perbc(LevV<T> l, T *x) {
