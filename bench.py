@@ -57,11 +57,16 @@ BYTES_PER_CELL = {
     "fcg_Ap": 4 + 8 + 12 + 16,                       # w, p, L -> p', q
     "fcg_update": 32 + 16,                           # p', q, x64, r64 -> x64, r64
     "redist": (12 + 16 + 16) / 3,                    # ϕ, ϕini (+ ϕ⁰ in stages 2, 3) -> ϕ', per stage
+    # single-reduction CG step on coarse level l (per cell of that level):
+    # r, w, s, iD, p, x, L -> p, s', x, r', w'
+    "cg1_l1": 36 + 20, "cg1_l2": 36 + 20, "cg1_l3": 36 + 20,
 }
+# coarse level of a class (its cells are (N >> level)^3); 0 for the fine-level classes
+LEVEL = {"cg1_l1": 1, "cg1_l2": 2, "cg1_l3": 3}
 # kernel classes timed with HIP events in the timed region; the one with the
 # largest total time is reported as the dominant kernel
 PROFILED = ["load_inc", "prolong_inc", "pcg_Ae", "pcg_last", "fcg_Ap", "fcg_update", "vof", "momentum",
-            "visc_update"]
+            "visc_update", "cg1_l1", "cg1_l2", "cg1_l3"]
 
 
 def pmc_traffic(kernel):
@@ -174,6 +179,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain / refined / learned solver comparison")
+    ap.add_argument("--no-events", action="store_true", help="time the steps without the per-launch HIP events")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the path shards by z-slabs across GPUs by design, "
@@ -188,7 +194,7 @@ def main():
     for _ in range(args.warmup):
         ia.sim_step_(sim)
     sim._call("ia_synchronize")
-    classes = PROFILED if args.profile == "auto" else [args.profile]
+    classes = [] if args.no_events else (PROFILED if args.profile == "auto" else [args.profile])
     for k in classes:
         sim._call("ia_profile_enable", ia.PROFILE[k], 1)
     iters, solve_ms, rel = [], [], []
@@ -209,14 +215,12 @@ def main():
         sim._call("ia_profile_enable", ia.PROFILE[k], 0)
         if nl.value:
             avg = ms.value / nl.value
+            kc = (N >> LEVEL.get(k, 0)) ** 3
             timed[k] = dict(launches=nl.value, total_ms=round(ms.value, 3), avg_ms=round(avg, 4),
-                            GBps=round(BYTES_PER_CELL[k] * cells / (avg * 1e-3) / 1e9, 1))
-    dom = max(timed, key=lambda k: timed[k]["total_ms"])
+                            GBps=round(BYTES_PER_CELL[k] * kc / (avg * 1e-3) / 1e9, 1), cells=kc)
+    dom = max(timed, key=lambda k: timed[k]["total_ms"]) if timed else None
     print(f"[bench] {args.steps} steps in {t:.3f} s; V-cycles per solve {iters}", file=sys.stderr, flush=True)
     value = cells * args.steps / t / 1e6
-    avg_ms = timed[dom]["avg_ms"]
-    bpl = BYTES_PER_CELL[dom] * cells
-    achieved = bpl / (avg_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -236,11 +240,16 @@ def main():
                    "cg_iterations_per_solve_mean": float(np.mean(iters)),
                    "poisson_time_to_1e-6_ms": round(float(np.mean(solve_ms)), 3),
                    "poisson_rel_residual_max": float(np.max(rel)) if rel else None},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic(dom),
-                     "bytes_per_launch": int(bpl), "avg_launch_ms": avg_ms, "launches": timed[dom]["launches"]},
         "kernels": timed,
     }
+    if dom:
+        avg_ms = timed[dom]["avg_ms"]
+        bpl = BYTES_PER_CELL[dom] * timed[dom]["cells"]
+        achieved = bpl / (avg_ms * 1e-3) / 1e9
+        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic(dom),
+                           "bytes_per_launch": int(bpl), "avg_launch_ms": avg_ms,
+                           "launches": timed[dom]["launches"]}
     out["levelset_redistance"] = redistance_rate(ia, sim, cells)
     sim.close()
     if not args.no_compare:

@@ -226,7 +226,10 @@ int ia_learned_grad(ia_ctx *c, const float *feat, const float *dout, float *gw);
 
 /* Kernel-class timing with HIP events on the context's stream (no reference
  * counterpart; used by the benchmark to measure one kernel's launches inside
- * a timed region).  Enabling resets the record; MG classes time level 0 only. */
+ * a timed region).  Enabling resets the record; MG classes time level 0 only,
+ * the CG1 classes the coarse level they name, with one event pair around each
+ * run of back-to-back step launches (launches counts the launches, total_ms
+ * the bracketed time, gaps between those launches included). */
 enum ia_prof_class {
     IA_PROF_VOF = 0,          /* fused cVOF sweep */
     IA_PROF_MOMENTUM = 1,     /* fused CMOM/SynDRoM sweep */
@@ -240,7 +243,10 @@ enum ia_prof_class {
     IA_PROF_FCG_UPDATE = 9,   /* FCG x += αp, r -= αq (FP64) */
     IA_PROF_PCG_LAST = 10,    /* last finest-level pcg! update fused with the FCG dots */
     IA_PROF_REDIST = 11,      /* one level-set reinitialisation stage (computeL! + update + BCf!) */
-    IA_PROF_NCLASS = 12
+    IA_PROF_CG1_L1 = 12,      /* single-reduction CG step of the FCG preconditioner, coarse level 1 */
+    IA_PROF_CG1_L2 = 13,      /* ... level 2 */
+    IA_PROF_CG1_L3 = 14,      /* ... level 3 */
+    IA_PROF_NCLASS = 15
 };
 int ia_profile_enable(ia_ctx *c, int kernel, int on);
 int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms);

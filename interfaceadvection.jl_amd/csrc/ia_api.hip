@@ -96,7 +96,8 @@ struct ia_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     struct ProfClass {
         bool on = false;
-        int n = 0;
+        int n = 0;         // event pairs
+        long launches = 0; // kernel launches inside them
         std::vector<hipEvent_t> ev;
     } prof[IA_PROF_NCLASS];
     char msg[512] = {0};
@@ -135,11 +136,14 @@ void prof_begin(ia_ctx *c, int k) {
     }
     (void)hipEventRecord(p.ev[2 * p.n], c->s);
 }
-void prof_end(ia_ctx *c, int k) {
+// nlaunch: kernel launches since the matching prof_begin (a bracket around a
+// run of back-to-back launches costs two events instead of two per launch)
+void prof_end(ia_ctx *c, int k, int nlaunch = 1) {
     auto &p = c->prof[k];
     if (!p.on || c->capturing) return;
     (void)hipEventRecord(p.ev[2 * p.n + 1], c->s);
     p.n++;
+    p.launches += nlaunch;
 }
 
 void grid_init(Grid &g, int D, const int *Ng, int perdir) {
@@ -452,6 +456,8 @@ template <class T> struct Impl {
         dim3 B = cb(l);
         T *r[2] = {P<T>(l.r), P<T>(l.r2)}, *w[2] = {P<T>(l.w), P<T>(l.w2)}, *s[2] = {P<T>(l.z), P<T>(l.z2)};
         hipLaunchKernelGGL(k_cg1_init<T>, B, dim3(NT), 0, c->s, v, w[0], c->pst, pp(c, PP_A));
+        int lv = (int)(&l - c->lev), pc = lv >= 1 && lv <= 3 ? IA_PROF_CG1_L1 + lv - 1 : -1;
+        if (pc >= 0) prof_begin(c, pc);
         for (int q = 1; q <= it; ++q) {
             int a = (q - 1) & 1;
             Cg1V<T> b{r[a], w[a], s[a], r[1 - a], w[1 - a], s[1 - a]};
@@ -463,6 +469,7 @@ template <class T> struct Impl {
                 hipLaunchKernelGGL(k_cg1_step<T>, B, dim3(NT), 0, c->s, v, b, c->pst, q, it, pin, pout);
             }
         }
+        if (pc >= 0) prof_end(c, pc, it);
     }
     static void vcycle(ia_ctx *c, int l, const int *cit, bool pre_done = false, bool then_pcg = false) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
@@ -1610,6 +1617,7 @@ int ia_profile_enable(ia_ctx *c, int kernel, int on) {
     if (!c || kernel < 0 || kernel >= IA_PROF_NCLASS) return IA_ERR_BADARG;
     c->prof[kernel].on = on != 0;
     c->prof[kernel].n = 0;
+    c->prof[kernel].launches = 0;
     return IA_OK;
 }
 int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms) {
@@ -1623,7 +1631,7 @@ int ia_profile_read(ia_ctx *c, int kernel, int *launches, double *total_ms) {
         HIPCHK(c, hipEventElapsedTime(&ms, p.ev[2 * q], p.ev[2 * q + 1]));
         t += ms;
     }
-    if (launches) *launches = p.n;
+    if (launches) *launches = (int)p.launches;
     if (total_ms) *total_ms = t;
     return IA_OK;
 }
