@@ -176,7 +176,7 @@ def main():
     ap.add_argument("--profile", default="auto", help="kernel class reported in `roofline` (default: the "
                     "event-timed class with the largest total time)")
     ap.add_argument("--cpu-N", type=int, default=192)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain / refined / learned solver comparison")
     ap.add_argument("--no-events", action="store_true", help="time the steps without the per-launch HIP events")
