@@ -88,7 +88,7 @@ def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     bl, kw = make_case(N_cpu, poisson)
     o = O.OracleSim((N_cpu,) * 3, (0.0, 0.0, 0.0), float(N_cpu), poisson="refined", poisson_rtol=rtol,
-                    smooth_it=1, **kw)
+                    smooth_it=2, **kw)
     s = pool_sdf_samples(N_cpu, bl, 0.7)
     o.lib.oc_apply_vof(o.ctx, O._ptr(s))
     o.Δt[-1] = min(o.Δt[-1], o.T.type(o.cfl()))

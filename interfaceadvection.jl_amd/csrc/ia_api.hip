@@ -65,7 +65,7 @@ struct ia_ctx {
     int use_graphs = 0;    // IA_USE_GRAPHS=1: replay each FCG iteration as a HIP graph
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
-    int smooth_it = 1;   // pcg! iterations on the finest level after each preconditioning V-cycle
+    int smooth_it = 2;   // pcg! iterations on the finest level after each preconditioning V-cycle
     // pcg! iterations on each coarse level inside Vcycle!: WaterLily's 6 (REFERENCE,
     // PLAIN), and fcg_coarse_its(level cells) in the flexible CG's preconditioner
     // (IA_FCG_CIT=a,b,... overrides levels 1, 2, ...; the last value repeats)
@@ -1372,6 +1372,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     c->dev = p->device;
     c->es = p->dtype == IA_F64 ? 8 : 4;
     if (const char *gv = getenv("IA_USE_GRAPHS")) c->use_graphs = atoi(gv) != 0;
+    if (const char *sv = getenv("IA_FCG_SMOOTH"); sv && *sv) c->smooth_it = std::max(0, std::min(CIT_MAX, atoi(sv)));
     for (int l = 0; l < MAXLEV; ++l) {
         long n = 1;
         for (int a = 0; a < p->D; ++a) n *= std::max(1L, (long)(p->N[a] >> l));

@@ -204,7 +204,7 @@ class OracleSim:
 
     def __init__(self, dims, u_BC, L=1.0, *, T=np.float32, λμ=1e-2, λρ=1e-3, ν=0.0, g=None,
                  perdir=(), Δt=0.25, InterfaceSDF=None, u0=None, normal="WH", limiter="Koren",
-                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=1, η=None):
+                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=2, η=None):
         self.T = np.dtype(T)
         self.lib = lib(self.T)
         self.dims = tuple(int(n) for n in dims)
@@ -289,7 +289,7 @@ class OracleSim:
         self.lib.oc_project_refined(self.ctx, float(dt_w), float(rtol), int(max_outer), C.byref(it), C.byref(rel))
         return it.value, rel.value
 
-    def project_fcg(self, dt_w, rtol=1e-6, max_it=200, smooth_it=6):
+    def project_fcg(self, dt_w, rtol=1e-6, max_it=200, smooth_it=2):
         it, rel = C.c_int(), C.c_double()
         self.lib.oc_project_fcg(self.ctx, float(dt_w), float(rtol), int(max_it), int(smooth_it), C.byref(it),
                                 C.byref(rel))

@@ -258,7 +258,7 @@ def test_project_refined_reaches_1e6(N):
     o.lib.oc_mg_update(o.ctx)
     st = ia.IASolveStats()
     g._call("ia_project", 1.0, ia.C.byref(st))
-    outer, rel = o.project_fcg(1.0, 1e-6, 100, 1)
+    outer, rel = o.project_fcg(1.0, 1e-6, 100, 2)
     assert st.rel <= 1e-6 and rel <= 1e-6
     assert abs(st.iters - outer) <= 3
     pg, po = O.inside(g.flow.p).astype(np.float64), O.inside(o.p).astype(np.float64)
@@ -289,7 +289,7 @@ def test_project_refined_3d(per, N):
     o.lib.oc_mg_update(o.ctx)
     st = ia.IASolveStats()
     g._call("ia_project", 1.0, ia.C.byref(st))
-    outer, rel = o.project_fcg(1.0, 1e-6, 100, 1)
+    outer, rel = o.project_fcg(1.0, 1e-6, 100, 2)
     print(f"{N}^3: gpu {st.iters} iterations (rel {st.rel:.2e}), oracle {outer} (rel {rel:.2e})")
     assert st.rel <= 1e-6 and rel <= 1e-6
     assert abs(st.iters - outer) <= 3
