@@ -328,20 +328,34 @@ template <int DIR, class T> __device__ inline T mom_face(const MomArgs<T> &a, in
     const T *__restrict__ us = a.us + g.ns * i;
     long Q = lin(g, q), sd = g.st[DIR];
     auto u = [&](int off) { return us[Q + off * sd]; };
-    if (q[DIR] == 1) {
-        if (!g.per[DIR]) {  // ϕuL
-            T c0 = u(0), cm = u(-1);
-            return Psi > 0 ? mom_phiq<DIR>(a, i, q, Psi, 2 * cm - c0, cm, c0) : mom_phiq<DIR>(a, i, q, Psi, u(1), c0, cm);
-        }
-        // ϕuP: the far-upwind value is the periodic image of I-2δ
-        return Psi > 0 ? mom_phiq<DIR>(a, i, q, Psi, u(n - 4), u(-1), u(0))
-                       : mom_phiq<DIR>(a, i, q, Psi, u(1), u(0), u(-1));
-    }
-    if (q[DIR] == n - 1) {  // ϕuR
+    // the limiter's (upwind, centre, downwind) triple for the sign of Psi,
+    // selected first so that a wave whose lanes differ in sign evaluates the
+    // face value once
+    T uu, cc, dd;
+    if (q[DIR] == 1 && !g.per[DIR]) {  // ϕuL
         T c0 = u(0), cm = u(-1);
-        return Psi < 0 ? mom_phiq<DIR>(a, i, q, Psi, 2 * c0 - cm, c0, cm) : mom_phiq<DIR>(a, i, q, Psi, u(-2), cm, c0);
+        bool pos = Psi > 0;
+        uu = pos ? 2 * cm - c0 : u(1);
+        cc = pos ? cm : c0;
+        dd = pos ? c0 : cm;
+    } else if (q[DIR] == 1) {  // ϕuP: the far-upwind value is the periodic image of I-2δ
+        bool pos = Psi > 0;
+        uu = pos ? u(n - 4) : u(1);
+        cc = pos ? u(-1) : u(0);
+        dd = pos ? u(0) : u(-1);
+    } else if (q[DIR] == n - 1) {  // ϕuR
+        T c0 = u(0), cm = u(-1);
+        bool neg = Psi < 0;
+        uu = neg ? 2 * c0 - cm : u(-2);
+        cc = neg ? c0 : cm;
+        dd = neg ? cm : c0;
+    } else {
+        bool pos = Psi > 0;
+        uu = pos ? u(-2) : u(1);
+        cc = pos ? u(-1) : u(0);
+        dd = pos ? u(0) : u(-1);
     }
-    return Psi > 0 ? mom_phiq<DIR>(a, i, q, Psi, u(-2), u(-1), u(0)) : mom_phiq<DIR>(a, i, q, Psi, u(1), u(0), u(-1));
+    return mom_phiq<DIR>(a, i, q, Psi, uu, cc, dd);
 }
 template <int D, int DIR, class T>
 __global__ void __launch_bounds__(NT) k_momentum_t(MomArgs<T> a) {
