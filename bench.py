@@ -107,13 +107,13 @@ def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
                        f"{t:.1f} s, CG iterations per solve {its}")
 
 
-def poisson_compare(ia, N, rtol, warm=8, steps=3, lr=1e-3):
+def poisson_compare(ia, N, rtol, warm=8, steps=3, lr=1e-3, case=None):
     """Time-to-rtol of the pressure solve for the plain multigrid, the flexible
     CG and the learned-correction modes on the same case and the same step
     range (the learned network trains during the warm-up steps, then is frozen)."""
     out = {}
     for mode in ("plain", "refined", "learned"):
-        sim, _, _ = build_sim(ia, N, mode, rtol)
+        sim, _, _ = build_sim(ia, N, mode, rtol, **(case or {}))
         if mode == "learned":
             if os.path.exists(WEIGHTS):  # trained by interfaceadvection.jl_amd/train_learned.py on other layouts
                 sim.learned_weights = np.fromfile(WEIGHTS, dtype="<f4")
@@ -170,7 +170,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--case", default="pool", choices=sorted(pool.CASES),
+                    help="pool: 256^3, 32 bubbles (the benchmark case); slab: 512^3, 256 bubbles")
+    ap.add_argument("--N", type=int, default=None, help="grid side (default: the case's)")
     ap.add_argument("--poisson", default="refined", choices=["refined", "plain", "reference", "learned"])
     ap.add_argument("--rtol", type=float, default=1e-6)
     ap.add_argument("--profile", default="auto", help="kernel class reported in `roofline` (default: the "
@@ -180,15 +182,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain / refined / learned solver comparison")
     ap.add_argument("--no-events", action="store_true", help="time the steps without the per-launch HIP events")
+    ap.add_argument("--compare-warm", type=int, default=8)
+    ap.add_argument("--compare-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the path shards by z-slabs across GPUs by design, "
               "but no multi-GPU run is part of it)", file=sys.stderr)
         sys.exit(2)
     ia = load_package()
-    N = args.N
+    case = dict(pool.CASES[args.case])
+    N = args.N or case.pop("N")
+    case.pop("N", None)
     t0 = time.perf_counter()
-    sim, bl, kw = build_sim(ia, N, args.poisson, args.rtol)
+    sim, bl, kw = build_sim(ia, N, args.poisson, args.rtol, **case)
     print(f"[bench] setup {N}^3 with {len(bl)} bubbles in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
     for _ in range(args.warmup):
@@ -234,8 +240,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"3D two-phase pool + {len(bl)} bubbles (seed 3), {N}^3, slip walls, gravity, "
-                               f"density ratio 1000, viscosity ratio 100",
+        "config": {"workload": f"3D two-phase pool (liquid below {case['zmax']} L) + {len(bl)} bubbles of radius "
+                               f"U({case['rmin']}, {case['rmax']}) L (seed {case['seed']}), {N}^3, slip walls, "
+                               f"gravity, density ratio 1000, viscosity ratio 100",
                    "grid": [N, N, N], "poisson": args.poisson, "poisson_rtol": args.rtol,
                    "cg_iterations_per_solve_mean": float(np.mean(iters)),
                    "poisson_time_to_1e-6_ms": round(float(np.mean(solve_ms)), 3),
@@ -247,13 +254,15 @@ def main():
         bpl = BYTES_PER_CELL[dom] * timed[dom]["cells"]
         achieved = bpl / (avg_ms * 1e-3) / 1e9
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
-                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": pmc_traffic(dom),
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                           "traffic": pmc_traffic(dom) if (args.case, N) == ("pool", 256) else None,
                            "bytes_per_launch": int(bpl), "avg_launch_ms": avg_ms,
                            "launches": timed[dom]["launches"]}
     out["levelset_redistance"] = redistance_rate(ia, sim, cells)
     sim.close()
     if not args.no_compare:
-        out["config"]["poisson_compare"] = poisson_compare(ia, N, args.rtol)
+        out["config"]["poisson_compare"] = poisson_compare(ia, N, args.rtol, warm=args.compare_warm,
+                                                           steps=args.compare_steps, case=case)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -1377,7 +1377,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
         long n = 1;
         for (int a = 0; a < p->D; ++a) n *= std::max(1L, (long)(p->N[a] >> l));
         c->cit_wl[l] = 6;
-        c->cit_fcg[l] = fcg_coarse_its(n);
+        c->cit_fcg[l] = fcg_coarse_its(n, l);
     }
     if (const char *cv = getenv("IA_FCG_CIT"); cv && *cv) {
         int l = 1, v = 0;

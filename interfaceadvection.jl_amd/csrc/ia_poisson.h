@@ -1304,13 +1304,14 @@ constexpr int MAXLEV = 12;
 // pcg! iterations per coarse level: at most CIT_MAX, three state slots each
 constexpr int CIT_MAX = 128;
 constexpr int PCG_SLOTS = 3 * CIT_MAX + 8;
-// the flexible CG preconditioner's pcg! iterations on a coarse level of n
-// interior cells.  Levels above 1M cells are bandwidth-bound and cost like
-// the fine level, so they keep WaterLily's 6; below, a pcg! iteration costs
-// little more than its launches, and solving the coarse problems more
-// exactly cuts the outer iterations (256^3 pool: 39 -> 12.6 per solve).
-__host__ __device__ inline int fcg_coarse_its(long n) {
-    return n > 1000000 ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
+// the flexible CG preconditioner's solver iterations on coarse level l of n
+// interior cells.  Level 1 above 1M cells is bandwidth-bound and costs like
+// the fine level, so it keeps WaterLily's 6; deeper levels cost little more
+// than their launches per iteration, and solving their problems more exactly
+// cuts the outer iterations (256^3 pool: 39 -> 9.8 per solve; 512^3 slab:
+// 32 -> 15 with 48 instead of 6 on its second level, 128^3).
+__host__ __device__ inline int fcg_coarse_its(long n, int l) {
+    return (l == 1 && n > 1000000) ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
 }
 constexpr size_t TAIL_LDS_MAX = 152 * 1024;  // of the 160 KiB of LDS per CU
 template <class T> struct TailLev {

@@ -1,12 +1,21 @@
-"""The 3D two-phase pool workload: a liquid pool below z = 0.7 L holding
-non-overlapping spherical bubbles, slip walls, gravity -z, density ratio 1000,
-viscosity ratio 100, L = N cells, U = 1, g0 = U^2/L, ν = UL/1e4.
+"""The 3D two-phase pool workloads: a liquid pool holding non-overlapping
+spherical bubbles, slip walls, gravity -z, density ratio 1000, viscosity ratio
+100, L = N cells, U = 1, g0 = U^2/L, ν = UL/1e4.
 
-Bubble radii are U(0.03, 0.08) L; centres are rejection-sampled uniformly in
-the pool with at least 2 cells of clearance to the walls, the free surface and
-each other (numpy PCG64 with the given seed).  The volume fractions come from
-applyVOF! (VOFutil.jl:8-37) on the SDF φ = max(z - 0.7 L, max_b(r_b - |x - c_b|)).
+- "pool": 256^3, liquid below z = 0.7 L, 32 bubbles of radius U(0.03, 0.08) L,
+  seed 3 (the benchmark case);
+- "slab": 512^3, liquid below z = 0.6 L, 256 bubbles of radius U(0.02, 0.05) L,
+  seed 5.
+
+Centres are rejection-sampled uniformly in the liquid with at least 2 cells
+of clearance to the walls, the free surface and each other (numpy PCG64 with
+the given seed).  The volume fractions come from applyVOF! (VOFutil.jl:8-37)
+on the SDF φ = max(z - zmax L, max_b(r_b - |x - c_b|)).
 """
+CASES = {
+    "pool": dict(N=256, nb=32, seed=3, rmin=0.03, rmax=0.08, zmax=0.7),
+    "slab": dict(N=512, nb=256, seed=5, rmin=0.02, rmax=0.05, zmax=0.6),
+}
 import numpy as np
 
 
@@ -66,20 +75,20 @@ def pool_sdf_samples(N, bl, zmax, T=np.float32):
     return np.ascontiguousarray(S.reshape(N * N * N, 1 + 2 * D, order="F"))
 
 
-def make_case(N, poisson, T=np.float32, nb=32, seed=3):
+def make_case(N, poisson, T=np.float32, nb=32, seed=3, rmin=0.03, rmax=0.08, zmax=0.7):
     L = float(N)
-    bl = bubbles(N, nb, seed, 0.03, 0.08, 0.7)
+    bl = bubbles(N, nb, seed, rmin, rmax, zmax)
     kw = dict(T=T, λρ=1e-3, λμ=1e-2, ν=1.0 * L / 1e4, g=(0.0, 0.0, -1.0 / L), perdir=())
     return bl, kw
 
 
-def build_sim(ia, N, poisson, rtol, seed=3):
+def build_sim(ia, N, poisson, rtol, seed=3, nb=32, rmin=0.03, rmax=0.08, zmax=0.7):
     """The pool workload at N^3 with the given Poisson mode, f initialised by
     applyVOF! from the SDF samples, Δt from MPCFL."""
-    bl, kw = make_case(N, poisson, seed=seed)
+    bl, kw = make_case(N, poisson, seed=seed, nb=nb, rmin=rmin, rmax=rmax, zmax=zmax)
     sim = ia.TwoPhaseSimulation((N, N, N), (0.0, 0.0, 0.0), float(N), poisson=poisson, poisson_rtol=rtol,
                                 poisson_max_outer=200, **kw)
-    s = pool_sdf_samples(N, bl, 0.7)
+    s = pool_sdf_samples(N, bl, zmax)
     sim._call("ia_apply_vof", ia._ptr(s), s.nbytes)
     del s
     sim.flow.Δt[-1] = min(sim.flow.Δt[-1], sim.T.type(sim.MPCFL()))

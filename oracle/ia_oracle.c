@@ -1572,12 +1572,12 @@ oc_ctx *oc_create(const oc_params *p) {
     c->mg_tol = p->mg_tol > 0 ? p->mg_tol : 1e-4;
     c->mg_itmx = p->mg_itmx > 0 ? p->mg_itmx : 200;
     c->coarse_it = 6;
-    /* the flexible CG preconditioner's pcg! iterations on a coarse level of n
-     * interior cells (the HIP library's fcg_coarse_its) */
+    /* the flexible CG preconditioner's solver iterations on coarse level l of
+     * n interior cells (the HIP library's fcg_coarse_its) */
     for (int l = 0; l < 16; ++l) {
         long n = 1;
         for (int a = 0; a < p->D; ++a) n *= (p->N[a] >> l) > 1 ? (p->N[a] >> l) : 1;
-        c->fcg_cit[l] = n > 1000000 ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
+        c->fcg_cit[l] = (l == 1 && n > 1000000) ? 6 : n > 100000 ? 48 : n > 8192 ? 64 : 32;
     }
     c->fcg_cg1 = 1;
     c->poisson_mode = p->poisson_mode;
