@@ -231,7 +231,7 @@ int ia_learned_grad(ia_ctx *c, const float *feat, const float *dout, float *gw);
  * run of back-to-back step launches (launches counts the launches, total_ms
  * the bracketed time, gaps between those launches included). */
 enum ia_prof_class {
-    IA_PROF_VOF = 0,          /* fused cVOF sweep */
+    IA_PROF_VOF = 0,          /* cVOF sweep (face pass + cell pass, one record per sweep) */
     IA_PROF_MOMENTUM = 1,     /* fused CMOM/SynDRoM sweep */
     IA_PROF_VISC_UPDATE = 2,  /* fused viscosity + RK update + coefficients */
     IA_PROF_PCG_AE = 3,       /* pcg z = Aϵ, finest level */

@@ -47,6 +47,7 @@ struct ia_ctx {
     int fi = 0, f0i = 1;
     void *u = nullptr, *u0 = nullptr, *ru = nullptr, *ru2 = nullptr, *ruf = nullptr, *F = nullptr;
     void *p = nullptr, *sig = nullptr, *L = nullptr, *al = nullptr, *nh = nullptr;
+    void *vface = nullptr;  // swept-volume face fluxes of the current VOF sweep
     int8_t *cb = nullptr;
     Level lev[12];
     int nlev = 0;
@@ -272,7 +273,7 @@ template <class T> struct Impl {
         void **vec[] = {&c->u, &c->u0, &c->ru, &c->ru2, &c->ruf, &c->F, &c->L, &c->nh};
         for (void **v : vec)
             if (int r = alloc_zero(c, v, nv * es)) return r;
-        void **sca[] = {&c->p, &c->sig, &c->al};
+        void **sca[] = {&c->p, &c->sig, &c->al, &c->vface};
         for (void **v : sca)
             if (int r = alloc_zero(c, v, ns * es)) return r;
         HIPCHK(c, hipMalloc(&c->cb, ns));
@@ -1077,8 +1078,9 @@ template <class T> struct Impl {
         a.ctol = 10 * (T)epsT();
         a.ubc_d = (T)c->P.u_bc[d];
         prof_begin(c, IA_PROF_VOF);
-        hipLaunchKernelGGL(k_vof_sweep<T>, blocks(c->t), dim3(NT), 0, c->s, a);
-        prof_end(c, IA_PROF_VOF);
+        hipLaunchKernelGGL(k_vof_face<T>, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->vface));
+        hipLaunchKernelGGL(k_vof_update<T>, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->vface));
+        prof_end(c, IA_PROF_VOF);  // one sweep (its two launches) per record
         hipLaunchKernelGGL(k_fill_reduce, dim3(FILL_R2), dim3(NT), 0, c->s, c->fpart, c->t.nblk, c->fpart + c->t.nblk);
         hipLaunchKernelGGL(k_fill_finalize<T>, dim3(1), dim3(NT), 0, c->s, g, c->fpart + c->t.nblk, (long)FILL_R2,
                            P<T>((void *)u1),
@@ -1425,7 +1427,7 @@ void ia_destroy(ia_ctx *c) {
     void *bufs[] = {c->fb[0], c->fb[1], c->fb[2], c->fb[3], c->u, c->u0, c->ru, c->ru2, c->ruf, c->F, c->p, c->sig,
                     c->st_buf, c->st_fb,
                     c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->pst, c->fst, c->pp, c->dsc, c->x64, c->r64, c->z64,
-                    c->pa64, c->pb64, c->q64};
+                    c->pa64, c->pb64, c->q64, c->vface};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int l = 0; l < c->nlev; ++l) {

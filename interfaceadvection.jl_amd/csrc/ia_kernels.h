@@ -93,9 +93,10 @@ __global__ void __launch_bounds__(NT) k_prepare(Grid g, Tiling t, const T *__res
 }
 
 // ---------------------------------------------------------------------------
-// one cVOF sweep along d: reconstruction of every donor cell on the fly,
-// swept-volume face fluxes, conservative update with the c̄ dilation term,
-// fill report, cleanWisp!, BCf! (advection.jl:80-137, normalEstimation.jl:10-28)
+// one cVOF sweep along d in two passes: the face pass reconstructs each
+// donor cell on the fly and computes every swept-volume face flux once; the
+// cell pass makes the conservative update with the c̄ dilation term, the fill
+// report, cleanWisp! and BCf! (advection.jl:80-137, normalEstimation.jl:10-28)
 // ---------------------------------------------------------------------------
 template <class T> struct SweepArgs {
     Grid g;
@@ -137,8 +138,42 @@ __device__ inline T vof_face_flux(const SweepArgs<T> &a, const int q[3], long IF
     return getVolumeFraction(g.D, nc, aa) * dl;
 }
 
+// pass 1 of a sweep: the swept-volume flux of every face along d, once (each
+// cell's lower face; the last cell also its upper face), into `face`, and the
+// mass flux ρuf of that face (advection.jl:108-137, flow.jl:207)
 template <class T>
-__global__ void __launch_bounds__(NT) k_vof_sweep(SweepArgs<T> a) {
+__global__ void __launch_bounds__(NT) k_vof_face(SweepArgs<T> a, T *__restrict__ face) {
+    const Grid &g = a.g;
+    int p[3];
+    if (!thread_cell(g, a.t, p)) return;
+    int d = a.d;
+    long I = lin(g, p), sd = g.st[d];
+    const T *u1 = a.u1, *u2 = a.u2;
+    bool nper = !g.per[d];
+    T dl0 = (a.dt / 2) * (u1[I] + u2[I]);
+    T ff0 = dl0 == T(0) ? T(0) : vof_face_flux(a, p, I, dl0);
+    face[I] = ff0;
+    T m0 = dl0 == T(0) ? T(0) : T(0) + (dl0 * a.lr + (1 - a.lr) * ff0);
+    if (a.mode == 0) {
+        m0 = m0 * a.idt;
+        if (nper && p[d] == 1) m0 = a.ubc_d;
+    }
+    a.ruf[I] = m0;
+    if (p[d] == g.Ng[d] - 2) {
+        int pu[3] = {p[0], p[1], p[2]};
+        pu[d] += 1;
+        T dl1 = (a.dt / 2) * (u1[I + sd] + u2[I + sd]);
+        T ff1 = dl1 == T(0) ? T(0) : vof_face_flux(a, pu, I + sd, dl1);
+        face[I + sd] = ff1;
+        T m1 = dl1 == T(0) ? T(0) : T(0) + (dl1 * a.lr + (1 - a.lr) * ff1);
+        if (a.mode == 0) m1 = nper ? a.ubc_d : m1 * a.idt;
+        a.ruf[I + sd] = m1;
+    }
+}
+// pass 2: the conservative update with the c̄ dilation term (advection.jl:83),
+// fill-report partials, cleanWisp! (VOFutil.jl:127-137) and BCf!
+template <class T>
+__global__ void __launch_bounds__(NT) k_vof_update(SweepArgs<T> a, const T *__restrict__ face) {
     const Grid &g = a.g;
     int p[3];
     bool act = thread_cell(g, a.t, p);
@@ -151,27 +186,7 @@ __global__ void __launch_bounds__(NT) k_vof_sweep(SweepArgs<T> a) {
         int d = a.d;
         long I = lin(g, p), sd = g.st[d];
         const T *u1 = a.u1, *u2 = a.u2;
-        // lower face (IFace = I) and upper face (IFace = I + δd)
-        T dl0 = (a.dt / 2) * (u1[I] + u2[I]);
-        T ff0 = dl0 == T(0) ? T(0) : vof_face_flux(a, p, I, dl0);
-        int pu[3] = {p[0], p[1], p[2]};
-        pu[d] += 1;
-        T dl1 = (a.dt / 2) * (u1[I + sd] + u2[I + sd]);
-        T ff1 = dl1 == T(0) ? T(0) : vof_face_flux(a, pu, I + sd, dl1);
-        // ρuf (advection.jl:127/:135 fᶠ2ρuf, flow.jl:207 rmul! + BC!)
-        bool nper = !g.per[d];
-        T m0 = dl0 == T(0) ? T(0) : T(0) + (dl0 * a.lr + (1 - a.lr) * ff0);
-        if (a.mode == 0) {
-            m0 = m0 * a.idt;
-            if (nper && p[d] == 1) m0 = a.ubc_d;
-        }
-        a.ruf[I] = m0;
-        if (p[d] == g.Ng[d] - 2) {
-            T m1 = dl1 == T(0) ? T(0) : T(0) + (dl1 * a.lr + (1 - a.lr) * ff1);
-            if (a.mode == 0) m1 = nper ? a.ubc_d : m1 * a.idt;
-            a.ruf[I + sd] = m1;
-        }
-        // update (advection.jl:83)
+        T ff0 = face[I], ff1 = face[I + sd];
         T dud = u1[I + sd] - u1[I], du0d = u2[I + sd] - u2[I];
         T fnew = a.fin[I] + ((ff0 - ff1) + (((T)a.cb[I] * (dud + du0d)) * a.dt) / 2);
         if (fnew != fnew) fp.nan = 1;
@@ -180,7 +195,6 @@ __global__ void __launch_bounds__(NT) k_vof_sweep(SweepArgs<T> a) {
             fp.vmin = (double)fnew;
             fp.imax = fp.imin = I;
         }
-        // cleanWisp! (VOFutil.jl:127-137) and BCf!
         T fc = fnew < a.ctol ? T(0) : (fnew > 1 - a.ctol ? T(1) : fnew);
         write_images<T>(g, a.fout, p, fc, 0, 0, T(0));
         if (a.fmid) write_images<T>(g, a.fmid, p, (fc + a.fn[I]) / 2, 0, 0, T(0));
