@@ -232,7 +232,7 @@ int ia_learned_grad(ia_ctx *c, const float *feat, const float *dout, float *gw);
  * the bracketed time, gaps between those launches included). */
 enum ia_prof_class {
     IA_PROF_VOF = 0,          /* cVOF sweep (face pass + cell pass, one record per sweep) */
-    IA_PROF_MOMENTUM = 1,     /* fused CMOM/SynDRoM sweep */
+    IA_PROF_MOMENTUM = 1,     /* CMOM/SynDRoM sweep (face pass + cell pass, one record per sweep) */
     IA_PROF_VISC_UPDATE = 2,  /* fused viscosity + RK update + coefficients */
     IA_PROF_PCG_AE = 3,       /* pcg z = Aϵ, finest level */
     IA_PROF_PCG_UPDATE = 4,   /* pcg x,r,z update, finest level */

@@ -48,6 +48,7 @@ struct ia_ctx {
     void *u = nullptr, *u0 = nullptr, *ru = nullptr, *ru2 = nullptr, *ruf = nullptr, *F = nullptr;
     void *p = nullptr, *sig = nullptr, *L = nullptr, *al = nullptr, *nh = nullptr;
     void *vface = nullptr;  // swept-volume face fluxes of the current VOF sweep
+    void *mface = nullptr;  // momentum face fluxes of the current sweep (components last)
     int8_t *cb = nullptr;
     Level lev[12];
     int nlev = 0;
@@ -270,7 +271,7 @@ template <class T> struct Impl {
         size_t ns = g.ns, nv = ns * g.D, es = sizeof(T);
         for (int q = 0; q < 4; ++q)
             if (int r = alloc_zero(c, &c->fb[q], ns * es)) return r;
-        void **vec[] = {&c->u, &c->u0, &c->ru, &c->ru2, &c->ruf, &c->F, &c->L, &c->nh};
+        void **vec[] = {&c->u, &c->u0, &c->ru, &c->ru2, &c->ruf, &c->F, &c->L, &c->nh, &c->mface};
         for (void **v : vec)
             if (int r = alloc_zero(c, v, nv * es)) return r;
         void **sca[] = {&c->p, &c->sig, &c->al, &c->vface};
@@ -1107,11 +1108,18 @@ template <class T> struct Impl {
         a.us = P<T>(c->nh);
         hipLaunchKernelGGL(k_ustar<T>, blocks1(g.ns), dim3(NT), 0, c->s, a);
         prof_begin(c, IA_PROF_MOMENTUM);
-        auto kern = k_momentum_t<3, 0, T>;
-        if (g.D == 3) kern = d == 0 ? k_momentum_t<3, 0, T> : d == 1 ? k_momentum_t<3, 1, T> : k_momentum_t<3, 2, T>;
-        else kern = d == 0 ? k_momentum_t<2, 0, T> : k_momentum_t<2, 1, T>;
-        hipLaunchKernelGGL(kern, blocks(c->t), dim3(NT), 0, c->s, a);
-        prof_end(c, IA_PROF_MOMENTUM);
+        auto kf = k_mom_face<3, 0, T>;
+        auto ku = k_mom_update<3, 0, T>;
+        if (g.D == 3) {
+            kf = d == 0 ? k_mom_face<3, 0, T> : d == 1 ? k_mom_face<3, 1, T> : k_mom_face<3, 2, T>;
+            ku = d == 0 ? k_mom_update<3, 0, T> : d == 1 ? k_mom_update<3, 1, T> : k_mom_update<3, 2, T>;
+        } else {
+            kf = d == 0 ? k_mom_face<2, 0, T> : k_mom_face<2, 1, T>;
+            ku = d == 0 ? k_mom_update<2, 0, T> : k_mom_update<2, 1, T>;
+        }
+        hipLaunchKernelGGL(kf, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->mface));
+        hipLaunchKernelGGL(ku, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->mface));
+        prof_end(c, IA_PROF_MOMENTUM);  // one sweep (its two launches) per record
         std::swap(c->ru, c->ru2);
     }
     static void dir_order(int D, int step_index, int *dirO) {
@@ -1427,7 +1435,7 @@ void ia_destroy(ia_ctx *c) {
     void *bufs[] = {c->fb[0], c->fb[1], c->fb[2], c->fb[3], c->u, c->u0, c->ru, c->ru2, c->ruf, c->F, c->p, c->sig,
                     c->st_buf, c->st_fb,
                     c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->pst, c->fst, c->pp, c->dsc, c->x64, c->r64, c->z64,
-                    c->pa64, c->pb64, c->q64, c->vface};
+                    c->pa64, c->pb64, c->q64, c->vface, c->mface};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int l = 0; l < c->nlev; ++l) {

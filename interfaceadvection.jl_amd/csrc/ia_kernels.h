@@ -371,18 +371,36 @@ template <int DIR, class T> __device__ inline T mom_face(const MomArgs<T> &a, in
     }
     return mom_phiq<DIR>(a, i, q, Psi, uu, cc, dd);
 }
+// the momentum sweep along DIR in two passes.  Face pass: the SynDRoM face
+// flux of every component at every face along DIR, once (each cell's lower
+// face; the last cell also its upper face), into `face` (components last)
 template <int D, int DIR, class T>
-__global__ void __launch_bounds__(NT) k_momentum_t(MomArgs<T> a) {
+__global__ void __launch_bounds__(NT) k_mom_face(MomArgs<T> a, T *__restrict__ face) {
     const Grid &g = a.g;
     int p[3];
     if (!thread_cell(g, a.t, p)) return;
-    long I = lin(g, p);
+    long I = lin(g, p), sd = g.st[DIR];
+    bool top = p[DIR] == g.Ng[DIR] - 2;
+    int pu[3] = {p[0], p[1], p[2]};
+    pu[DIR] += 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        face[g.ns * i + I] = mom_face<DIR>(a, i, p);
+        if (top) face[g.ns * i + I + sd] = mom_face<DIR>(a, i, pu);
+    }
+}
+// cell pass: the two-pass difference of the face fluxes, the uOld dilation
+// term and ρu += δt·r (flow.jl:216-231)
+template <int D, int DIR, class T>
+__global__ void __launch_bounds__(NT) k_mom_update(MomArgs<T> a, const T *__restrict__ face) {
+    const Grid &g = a.g;
+    int p[3];
+    if (!thread_cell(g, a.t, p)) return;
+    long I = lin(g, p), sd = g.st[DIR];
     T dl0 = mom_dila<DIR>(a, p);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-        int pu[3] = {p[0], p[1], p[2]};
-        pu[DIR] += 1;
-        T flo = mom_face<DIR>(a, i, p), fup = mom_face<DIR>(a, i, pu);
+        T flo = face[g.ns * i + I], fup = face[g.ns * i + I + sd];
         int pm[3] = {p[0], p[1], p[2]};
         pm[i] -= 1;
         T r = (T(0) + flo) - fup;
