@@ -195,6 +195,11 @@ def main():
     case.pop("N", None)
     t0 = time.perf_counter()
     sim, bl, kw = build_sim(ia, N, args.poisson, args.rtol, **case)
+    if args.poisson == "learned":  # the shipped parameters, frozen
+        if os.path.exists(WEIGHTS):
+            sim.learned_weights = np.fromfile(WEIGHTS, dtype="<f4")
+        else:
+            sim.learned_init(seed=0)
     print(f"[bench] setup {N}^3 with {len(bl)} bubbles in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
     for _ in range(args.warmup):

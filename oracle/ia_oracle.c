@@ -894,7 +894,7 @@ static void reconstruct(oc_ctx *c, T *f, T *al, T *nh) {
             al[I] = 0;
         } else {
             normal_scheme(g, c->scheme, f, nh, I);
-            T n[3];
+            T n[3] = {0, 0, 0};
             for (int d = 1; d <= g->D; ++d) n[d - 1] = VC(nh, d)[I];
             al[I] = getInterceptN(g->D, n, f[I]);
         }
