@@ -175,8 +175,11 @@ CTiling ctiling(const Grid &g) {
     t.nt[0] = (n0 + TX - 1) / TX;
     t.nt[1] = (n1 + TY - 1) / TY;
     long base = (long)t.nt[0] * t.nt[1];
+    // cells per thread column: at most 32, and no fewer workgroups than about
+    // 1024 (128³: 8-cell columns, 1024 workgroups; measured faster than 4- or
+    // 16-cell columns on the coarse CG there)
     int zb = 1;
-    if (g.D > 2) zb = (int)std::max(1L, std::min(32L, base * n2 / 2048));
+    if (g.D > 2) zb = (int)std::max(1L, std::min(32L, base * n2 / 1024));
     t.zb = zb;
     t.nt[2] = g.D > 2 ? (n2 + zb - 1) / zb : 1;
     t.nblk = base * t.nt[2];
@@ -1079,7 +1082,8 @@ template <class T> struct Impl {
         a.ctol = 10 * (T)epsT();
         a.ubc_d = (T)c->P.u_bc[d];
         prof_begin(c, IA_PROF_VOF);
-        hipLaunchKernelGGL(k_vof_face<T>, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->vface));
+        long nfb = c->t.nblk + (d == 0 ? top_face_x_blocks(g) : 0);
+        hipLaunchKernelGGL(k_vof_face<T>, dim3((unsigned)nfb), dim3(NT), 0, c->s, a, P<T>(c->vface));
         hipLaunchKernelGGL(k_vof_update<T>, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->vface));
         prof_end(c, IA_PROF_VOF);  // one sweep (its two launches) per record
         hipLaunchKernelGGL(k_fill_reduce, dim3(FILL_R2), dim3(NT), 0, c->s, c->fpart, c->t.nblk, c->fpart + c->t.nblk);
@@ -1117,7 +1121,8 @@ template <class T> struct Impl {
             kf = d == 0 ? k_mom_face<2, 0, T> : k_mom_face<2, 1, T>;
             ku = d == 0 ? k_mom_update<2, 0, T> : k_mom_update<2, 1, T>;
         }
-        hipLaunchKernelGGL(kf, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->mface));
+        long nfb = c->t.nblk + (d == 0 ? top_face_x_blocks(g) : 0);
+        hipLaunchKernelGGL(kf, dim3((unsigned)nfb), dim3(NT), 0, c->s, a, P<T>(c->mface));
         hipLaunchKernelGGL(ku, blocks(c->t), dim3(NT), 0, c->s, a, P<T>(c->mface));
         prof_end(c, IA_PROF_MOMENTUM);  // one sweep (its two launches) per record
         std::swap(c->ru, c->ru2);

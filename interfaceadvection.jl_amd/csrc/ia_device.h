@@ -58,6 +58,22 @@ __device__ inline bool thread_cell(const Grid &g, const Tiling &t, int p[3]) {
 __device__ inline long lin(const Grid &g, const int p[3]) {
     return (long)p[0] + g.st[1] * p[1] + g.st[2] * p[2];
 }
+// upper x-face q of the interior (one per x-row, rows in y-then-z order):
+// the cell index of that face, x = Ng[0]-1.  A face pass along x computes
+// these in blocks of their own, since inside the tiling they fall on one lane
+// of a wavefront and would serialise it.
+__device__ inline bool top_face_x(const Grid &g, long q, int p[3]) {
+    long n1 = g.D > 1 ? g.Ng[1] - 2 : 1, n2 = g.D > 2 ? g.Ng[2] - 2 : 1;
+    if (q >= n1 * n2) return false;
+    p[0] = g.Ng[0] - 1;
+    p[1] = g.D > 1 ? 1 + (int)(q % n1) : 0;
+    p[2] = g.D > 2 ? 1 + (int)(q / n1) : 0;
+    return true;
+}
+__host__ inline long top_face_x_blocks(const Grid &g) {
+    long n1 = g.D > 1 ? g.Ng[1] - 2 : 1, n2 = g.D > 2 ? g.Ng[2] - 2 : 1;
+    return (n1 * n2 + TX * TY - 1) / (TX * TY);
+}
 
 // ---------------------------------------------------------------------------
 // boundary maps.  Reading through these equals reading the reference's array

@@ -141,33 +141,36 @@ __device__ inline T vof_face_flux(const SweepArgs<T> &a, const int q[3], long IF
 // pass 1 of a sweep: the swept-volume flux of every face along d, once (each
 // cell's lower face; the last cell also its upper face), into `face`, and the
 // mass flux ρuf of that face (advection.jl:108-137, flow.jl:207)
+// the flux through the lower d-face of cell p (flat index I)
+template <class T> __device__ inline void vof_face_one(const SweepArgs<T> &a, T *__restrict__ face, const int p[3],
+                                                       long I) {
+    const Grid &g = a.g;
+    int d = a.d;
+    T dl = (a.dt / 2) * (a.u1[I] + a.u2[I]);
+    T ff = dl == T(0) ? T(0) : vof_face_flux(a, p, I, dl);
+    face[I] = ff;
+    T m = dl == T(0) ? T(0) : T(0) + (dl * a.lr + (1 - a.lr) * ff);
+    if (a.mode == 0) m = !g.per[d] && (p[d] == 1 || p[d] == g.Ng[d] - 1) ? a.ubc_d : m * a.idt;
+    a.ruf[I] = m;
+}
 template <class T>
 __global__ void __launch_bounds__(NT) k_vof_face(SweepArgs<T> a, T *__restrict__ face) {
     const Grid &g = a.g;
     int p[3];
+    if (a.d == 0 && (long)blockIdx.x >= a.t.nblk) {
+        // the blocks past the tiling: the upper x-face of each x-row
+        if (!top_face_x(g, (long)(blockIdx.x - a.t.nblk) * NT + threadIdx.x, p)) return;
+        vof_face_one(a, face, p, lin(g, p));
+        return;
+    }
     if (!thread_cell(g, a.t, p)) return;
     int d = a.d;
-    long I = lin(g, p), sd = g.st[d];
-    const T *u1 = a.u1, *u2 = a.u2;
-    bool nper = !g.per[d];
-    T dl0 = (a.dt / 2) * (u1[I] + u2[I]);
-    T ff0 = dl0 == T(0) ? T(0) : vof_face_flux(a, p, I, dl0);
-    face[I] = ff0;
-    T m0 = dl0 == T(0) ? T(0) : T(0) + (dl0 * a.lr + (1 - a.lr) * ff0);
-    if (a.mode == 0) {
-        m0 = m0 * a.idt;
-        if (nper && p[d] == 1) m0 = a.ubc_d;
-    }
-    a.ruf[I] = m0;
-    if (p[d] == g.Ng[d] - 2) {
+    long I = lin(g, p);
+    vof_face_one(a, face, p, I);
+    if (d != 0 && p[d] == g.Ng[d] - 2) {
         int pu[3] = {p[0], p[1], p[2]};
         pu[d] += 1;
-        T dl1 = (a.dt / 2) * (u1[I + sd] + u2[I + sd]);
-        T ff1 = dl1 == T(0) ? T(0) : vof_face_flux(a, pu, I + sd, dl1);
-        face[I + sd] = ff1;
-        T m1 = dl1 == T(0) ? T(0) : T(0) + (dl1 * a.lr + (1 - a.lr) * ff1);
-        if (a.mode == 0) m1 = nper ? a.ubc_d : m1 * a.idt;
-        a.ruf[I + sd] = m1;
+        vof_face_one(a, face, pu, I + g.st[d]);
     }
 }
 // pass 2: the conservative update with the c̄ dilation term (advection.jl:83),
@@ -378,9 +381,17 @@ template <int D, int DIR, class T>
 __global__ void __launch_bounds__(NT) k_mom_face(MomArgs<T> a, T *__restrict__ face) {
     const Grid &g = a.g;
     int p[3];
+    if (DIR == 0 && (long)blockIdx.x >= a.t.nblk) {
+        // the blocks past the tiling: the upper x-face of each x-row
+        if (!top_face_x(g, (long)(blockIdx.x - a.t.nblk) * NT + threadIdx.x, p)) return;
+        long It = lin(g, p);
+#pragma unroll
+        for (int i = 0; i < D; ++i) face[g.ns * i + It] = mom_face<DIR>(a, i, p);
+        return;
+    }
     if (!thread_cell(g, a.t, p)) return;
     long I = lin(g, p), sd = g.st[DIR];
-    bool top = p[DIR] == g.Ng[DIR] - 2;
+    bool top = DIR != 0 && p[DIR] == g.Ng[DIR] - 2;
     int pu[3] = {p[0], p[1], p[2]};
     pu[DIR] += 1;
 #pragma unroll
@@ -445,31 +456,74 @@ __global__ void __launch_bounds__(NT) k_fface(ViscArgs<T> a, T *ff) {
     q[2] = (int)(r / g.Ng[1]);
     for (int c = 0; c < g.D; ++c) ff[g.ns * c + K] = visc_ff(a, c, q);
 }
-template <class T> __device__ inline T viscF(const ViscArgs<T> &a, int i, int j, const int qin[3]) {
-    const Grid &g = a.g;
-    int q[3] = {qin[0], qin[1], qin[2]};
-    if (g.per[j] && q[j] == g.Ng[j] - 1) q[j] = 1;  // flow.jl:153
-    long Q = lin(g, q);
-    const T *__restrict__ ffi = a.ff + g.ns * i;
-    const T *__restrict__ ffj = a.ff + g.ns * j;
-    T f1 = ffi[Q - g.st[j]], f2 = ffi[Q], f3 = ffj[Q - g.st[i]], f4 = ffj[Q];
+// viscF(i, j, Q) (flow.jl:144, getμ VOFutil.jl:186-191) on loaded values:
+// f1..f4 the face fractions ff_i[Q-δj], ff_i[Q], ff_j[Q-δi], ff_j[Q]; then
+// u_i[Q], u_i[Q-δj], u_j[Q], u_j[Q-δi]
+template <class T>
+__device__ inline T viscF_v(const ViscArgs<T> &a, T f1, T f2, T f3, T f4, T ui, T uim, T uj, T ujm) {
     T s = (((f1 + f2) + f3) + f4) / 4;
     T fr = a.lr < T(1) ? tmin(tmin(tmin(f1, f2), f3), f4) : tmax(tmax(tmax(f1, f2), f3), f4);
     T mu = a.mu * tmin(linp(s, a.lm), (s > T(0.5) ? T(1) : a.lm / a.lr) * linp(fr, a.lr));
-    const T *ui = a.u + g.ns * i, *uj = a.u + g.ns * j;
-    return mu * ((ui[Q] - ui[Q - g.st[j]]) + (uj[Q] - uj[Q - g.st[i]]));
+    return mu * ((ui - uim) + (uj - ujm));
 }
-template <int D, class T> __device__ inline T visc_force(const ViscArgs<T> &a, int i, const int p[3]) {
-    if (!a.has_mu) return T(0);
-    T r = 0;
+// the viscous force of every component of cell p (visc!, flow.jl:119-138):
+// F_i = Σ_j -viscF(i,j,p) + viscF(i,j,p+δj), with each value loaded once.
+// The D² viscF at p share the face fractions and velocities of p and its
+// lower neighbours; viscF(i,j,p+δj) reads p's values again at its lower
+// j-side (q wraps to the first interior cell on a periodic axis, flow.jl:153).
+template <int D, class T> __device__ inline void visc_force_all(const ViscArgs<T> &a, const int p[3], T F[D]) {
+    if (!a.has_mu) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) F[i] = T(0);
+        return;
+    }
+    const Grid &g = a.g;
+    const T *__restrict__ ff = a.ff;
+    const T *__restrict__ u = a.u;
+    const long ns = g.ns, I = lin(g, p);
+    T ffc[D], uc[D], ffm[D][D], um[D][D];  // [c][k]: component c at I - δk
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        ffc[c] = ff[ns * c + I];
+        uc[c] = u[ns * c + I];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            ffm[c][k] = ff[ns * c + I - g.st[k]];
+            um[c][k] = u[ns * c + I - g.st[k]];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) F[i] = 0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        int pu[3] = {p[0], p[1], p[2]};
-        pu[j] += 1;
-        r += -viscF(a, i, j, p);
-        r -= -viscF(a, i, j, pu);
+        // q = p + δj, wrapped to the first interior cell on a periodic axis (flow.jl:153)
+        bool wrap = g.per[j] && p[j] + 1 == g.Ng[j] - 1;
+        long Q = wrap ? I - (long)(g.Ng[j] - 3) * g.st[j] : I + g.st[j];
+        T ffq[D], uq[D], ffjm[D], ujm[D];  // at Q: every component; component j at Q - δi
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            ffq[c] = ff[ns * c + Q];
+            uq[c] = u[ns * c + Q];
+        }
+        T ffb[D], ub[D];  // every component at Q - δj (p itself unless wrapped)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            ffb[c] = wrap ? ff[ns * c + Q - g.st[j]] : ffc[c];
+            ub[c] = wrap ? u[ns * c + Q - g.st[j]] : uc[c];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            ffjm[i] = i == j ? ffb[j] : ff[ns * j + Q - g.st[i]];
+            ujm[i] = i == j ? ub[j] : u[ns * j + Q - g.st[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            T lo = viscF_v(a, ffm[i][j], ffc[i], ffm[j][i], ffc[j], uc[i], um[i][j], uc[j], um[j][i]);
+            T hi = viscF_v(a, ffb[i], ffq[i], ffjm[i], ffq[j], uq[i], ub[i], uq[j], ujm[i]);
+            F[i] += -lo;
+            F[i] -= -hi;
+        }
     }
-    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -674,10 +728,14 @@ __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
     if (!thread_cell(g, a.v.t, p)) return;
     long I = lin(g, p);
     T wT = a.w, ac = T(1) / wT - 1;
+    // every component's stress divergence first: its loads then share one
+    // another's addresses, as no store intervenes
+    T Fv[D];
+    visc_force_all<D>(a.v, p, Fv);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         long o = g.ns * i + I;
-        T F = visc_force<D>(a.v, i, p);
+        T F = Fv[i];
         if (a.st) F = F + a.st[o];
         T ru0 = a.u0[o] * linp((a.fcur[I] + a.fcur[I - g.st[i]]) / 2, a.v.lr);
         T run = ((ac * ru0 + a.ru[o]) + F * a.dt) * wT;
@@ -695,8 +753,10 @@ __global__ void __launch_bounds__(NT) k_visc_force(ViscArgs<T> a, T *F) {
     int p[3];
     if (!thread_cell(g, a.t, p)) return;
     long I = lin(g, p);
+    T Fv[D];
+    visc_force_all<D>(a, p, Fv);
 #pragma unroll
-    for (int i = 0; i < D; ++i) F[g.ns * i + I] = visc_force<D>(a, i, p);
+    for (int i = 0; i < D; ++i) F[g.ns * i + I] = Fv[i];
 }
 // updateL! alone (stage entry point)
 template <class T>
