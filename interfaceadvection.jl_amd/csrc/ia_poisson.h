@@ -1326,16 +1326,20 @@ template <class T> struct TailArgs {
     int l0, nlev;
     const int *halt;
 };
-__device__ inline double bsum1024(double v, double *sh) {
+// workgroup sum (every thread gets it) with one barrier: the per-wave sums
+// alternate between two halves of sh, so a wave writing reduction k+1's
+// half cannot overtake a wave still reading reduction k's -- it has passed
+// reduction k+1's barrier only after every wave finished reading k-1's half.
+__device__ inline double bsum1024(double v, double *sh, int &par) {
     v = wave_sum(v);
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) sh[w] = v;
+    double *h = sh + par * (NT1 / 64);
+    par ^= 1;
+    if (lane == 0) h[w] = v;
     __syncthreads();
     double s = 0;
-    for (int q = 0; q < NT1 / 64; ++q) s += sh[q];
-    __syncthreads();
-    return s;  // every thread
+    for (int q = 0; q < NT1 / 64; ++q) s += h[q];
+    return s;
 }
 // ---------------------------------------------------------------------------
 // The coarsest levels in one workgroup with every array resident in LDS.
@@ -1390,12 +1394,17 @@ __device__ inline void scoord(const int n[3], int c, int q[3]) {
     q[1] = r % n[1];
     q[2] = r / n[1];
 }
-#define SLOOP(S, ...)                                        \
+// every cell of a level, each thread the cells threadIdx.x + k*NT1; SLOOP
+// then waits for the workgroup, SLOOP_OWN does not (for a loop whose cells'
+// values the next phase reads only at the same cells, i.e. by the same thread)
+#define SLOOP_OWN(S, ...)                                    \
     for (int c = threadIdx.x; c < (S).ncell; c += NT1) {     \
         int q[3];                                            \
         scoord((S).n, c, q);                                 \
         __VA_ARGS__                                          \
-    }                                                        \
+    }
+#define SLOOP(S, ...)       \
+    SLOOP_OWN(S, __VA_ARGS__) \
     __syncthreads();
 
 template <int D, class T> __device__ void stail_increment(const SLev<T> &l) {
@@ -1404,27 +1413,31 @@ template <int D, class T> __device__ void stail_increment(const SLev<T> &l) {
         l.x[c] = l.x[c] + l.e[c];
     })
 }
-template <int D, class T> __device__ void stail_pcg(const SLev<T> &l, int it, double *sh) {
+// pcg!(l; it).  Three barriers per iteration: the two sums, and the one
+// before A·ϵ reads the neighbours' ϵ; the other phases touch only each
+// thread's own cells.  Every exit is past a barrier that follows the last
+// write of x, so the caller may read any cell's x.
+template <int D, class T> __device__ void stail_pcg(const SLev<T> &l, int it, double *sh, int &par) {
     double v = 0;
-    SLOOP(l, {
+    SLOOP_OWN(l, {
         T zz = l.r[c] * l.iD[c];
         l.z[c] = zz;
         l.e[c] = zz;
         v += (double)l.r[c] * (double)zz;
     })
-    T rho = (T)bsum1024(v, sh);
+    T rho = (T)bsum1024(v, sh, par);  // also orders the ϵ writes before A·ϵ
     if (fabs(rho) < 10 * eps_of<T>()) return;
     for (int i = 1; i <= it; ++i) {
         v = 0;
-        SLOOP(l, {
+        SLOOP_OWN(l, {
             T zz = smult<D>(l, l.e, c, q);
             l.z[c] = zz;
             v += (double)zz * (double)l.e[c];
         })
-        T alpha = rho / (T)bsum1024(v, sh);
+        T alpha = rho / (T)bsum1024(v, sh, par);
         bool last = i == it || fabs((double)alpha) < 1e-2;
         v = 0;
-        SLOOP(l, {
+        SLOOP_OWN(l, {
             l.x[c] += alpha * l.e[c];
             T rr = l.r[c] - alpha * l.z[c];
             l.r[c] = rr;
@@ -1432,8 +1445,11 @@ template <int D, class T> __device__ void stail_pcg(const SLev<T> &l, int it, do
             l.z[c] = zz;
             v += (double)rr * (double)zz;
         })
-        if (last) return;
-        T rho2 = (T)bsum1024(v, sh);
+        if (last) {
+            __syncthreads();
+            return;
+        }
+        T rho2 = (T)bsum1024(v, sh, par);
         if (fabs(rho2) < 10 * eps_of<T>()) return;
         T beta = rho2 / rho;
         SLOOP(l, { l.e[c] = beta * l.e[c] + l.z[c]; })
@@ -1472,6 +1488,7 @@ template <class T> __device__ inline SLev<T> slev(const TailArgs<T> &a, int l, T
     return s;
 }
 template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem, double *sh) {
+    int par = 0;
     for (int l = a.l0; l < a.nlev; ++l) {
         const TailLev<T> &G = a.lv[l];
         SLev<T> s = slev(a, l, smem);
@@ -1494,7 +1511,9 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
         SLev<T> f = slev(a, l, smem), cs = slev(a, l + 1, smem);
         SLOOP(f, { f.e[c] = f.r[c] * f.iD[c]; })
         stail_increment<D>(f);
-        SLOOP(cs, {
+        // each thread then reads, in the next Jacobi! or pcg!, only the
+        // coarse cells it restricts here
+        SLOOP_OWN(cs, {
             T s = 0;
             for (int c2 = (cs.D > 2 ? 2 * q[2] : 0); c2 <= (cs.D > 2 ? 2 * q[2] + 1 : 0); ++c2)
                 for (int c1 = 2 * q[1]; c1 <= 2 * q[1] + 1; ++c1)
@@ -1506,7 +1525,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
     // ascend: pcg!(l+1), prolongate!, increment!(l)
     for (int l = last - 1; l >= a.l0; --l) {
         SLev<T> f = slev(a, l, smem), cs = slev(a, l + 1, smem);
-        stail_pcg<D>(cs, a.cit[l + 1], sh);
+        stail_pcg<D>(cs, a.cit[l + 1], sh, par);
         SLOOP(f, {
             int k0 = q[0] / 2, k1 = f.D > 1 ? q[1] / 2 : 0, k2 = f.D > 2 ? q[2] / 2 : 0;
             f.e[c] = cs.x[k0 + cs.n[0] * (k1 + cs.n[1] * k2)];
@@ -1514,7 +1533,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
         stail_increment<D>(f);
     }
     SLev<T> s = slev(a, a.l0, smem);
-    stail_pcg<D>(s, a.cit[a.l0], sh);
+    stail_pcg<D>(s, a.cit[a.l0], sh, par);
     // the top level's solution back to global memory (interior)
     const TailLev<T> &G = a.lv[a.l0];
     SLOOP(s, {
@@ -1524,7 +1543,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
 }
 template <class T> __global__ void __launch_bounds__(NT1) k_mg_tail(TailArgs<T> a) {
     HALT_IF(a)
-    __shared__ double sh[NT1 / 64];
+    __shared__ double sh[2 * (NT1 / 64)];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = (T *)smem_raw;
     if (a.lv[a.l0].g.D == 3) tail_run<3>(a, smem, sh);
