@@ -336,7 +336,7 @@ template <class T> struct Impl {
             for (int l = 0; l < c->nlev && l < MAXLEV; ++l) gs[l] = c->lev[l].g;
             c->tail_level = c->nlev;
             for (int l = 1; l < c->nlev && l < MAXLEV; ++l)
-                if (interior(c->lev[l].g) <= 4096 && tail_lds_bytes<T>(gs, l, c->nlev) <= TAIL_LDS_MAX) {
+                if (interior(c->lev[l].g) <= TAIL_KC * NT1 && tail_lds_bytes<T>(gs, l, c->nlev) <= TAIL_LDS_MAX) {
                     c->tail_level = l;
                     break;
                 }

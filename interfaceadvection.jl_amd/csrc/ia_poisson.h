@@ -1413,48 +1413,93 @@ template <int D, class T> __device__ void stail_increment(const SLev<T> &l) {
         l.x[c] = l.x[c] + l.e[c];
     })
 }
-// pcg!(l; it).  Three barriers per iteration: the two sums, and the one
-// before A·ϵ reads the neighbours' ϵ; the other phases touch only each
-// thread's own cells.  Every exit is past a barrier that follows the last
-// write of x, so the caller may read any cell's x.
+// pcg!(l; it).  A thread's cells are threadIdx.x + k*NT1 (at most TAIL_KC
+// of them), and their x, r, z, ϵ and iD stay in its registers for the whole
+// solve: only ϵ, which A·ϵ reads at the neighbours, is also written to LDS.
+// Three barriers per iteration: the two sums, and the one after the ϵ update
+// (the first sum orders the initial ϵ).  The arithmetic is that of pcg!'s
+// global-memory kernels, term for term.  On exit x and r are written back,
+// past a barrier, so the caller may read any cell's x.
+constexpr int TAIL_KC = 4;  // 4096-cell levels over NT1 threads
 template <int D, class T> __device__ void stail_pcg(const SLev<T> &l, int it, double *sh, int &par) {
+    int cc[TAIL_KC], q[TAIL_KC][3];
+    bool on[TAIL_KC];
+    T x[TAIL_KC], r[TAIL_KC], z[TAIL_KC], e[TAIL_KC], iD[TAIL_KC];
     double v = 0;
-    SLOOP_OWN(l, {
-        T zz = l.r[c] * l.iD[c];
-        l.z[c] = zz;
-        l.e[c] = zz;
-        v += (double)l.r[c] * (double)zz;
-    })
-    T rho = (T)bsum1024(v, sh, par);  // also orders the ϵ writes before A·ϵ
-    if (fabs(rho) < 10 * eps_of<T>()) return;
+#pragma unroll
+    for (int k = 0; k < TAIL_KC; ++k) {
+        cc[k] = threadIdx.x + k * NT1;
+        on[k] = cc[k] < l.ncell;
+        x[k] = r[k] = z[k] = e[k] = iD[k] = T(0);
+        q[k][0] = q[k][1] = q[k][2] = 0;
+        if (on[k]) {
+            int c = cc[k];
+            scoord(l.n, c, q[k]);
+            x[k] = l.x[c];
+            r[k] = l.r[c];
+            iD[k] = l.iD[c];
+            T zz = r[k] * iD[k];
+            z[k] = zz;
+            e[k] = zz;
+            l.e[c] = zz;
+            v += (double)r[k] * (double)zz;
+        }
+    }
+    auto write_back = [&]() {
+#pragma unroll
+        for (int k = 0; k < TAIL_KC; ++k)
+            if (on[k]) {
+                l.x[cc[k]] = x[k];
+                l.r[cc[k]] = r[k];
+            }
+        __syncthreads();
+    };
+    T rho = (T)bsum1024(v, sh, par);
+    if (fabs(rho) < 10 * eps_of<T>()) return;  // x and r unchanged
     for (int i = 1; i <= it; ++i) {
         v = 0;
-        SLOOP_OWN(l, {
-            T zz = smult<D>(l, l.e, c, q);
-            l.z[c] = zz;
-            v += (double)zz * (double)l.e[c];
-        })
+        T zz[TAIL_KC];
+#pragma unroll
+        for (int k = 0; k < TAIL_KC; ++k) zz[k] = on[k] ? smult<D>(l, l.e, cc[k], q[k]) : T(0);
+#pragma unroll
+        for (int k = 0; k < TAIL_KC; ++k)
+            if (on[k]) {
+                z[k] = zz[k];
+                v += (double)zz[k] * (double)e[k];
+            }
         T alpha = rho / (T)bsum1024(v, sh, par);
         bool last = i == it || fabs((double)alpha) < 1e-2;
         v = 0;
-        SLOOP_OWN(l, {
-            l.x[c] += alpha * l.e[c];
-            T rr = l.r[c] - alpha * l.z[c];
-            l.r[c] = rr;
-            T zz = rr * l.iD[c];
-            l.z[c] = zz;
-            v += (double)rr * (double)zz;
-        })
+#pragma unroll
+        for (int k = 0; k < TAIL_KC; ++k)
+            if (on[k]) {
+                x[k] += alpha * e[k];
+                T rr = r[k] - alpha * z[k];
+                r[k] = rr;
+                T z2 = rr * iD[k];
+                z[k] = z2;
+                v += (double)rr * (double)z2;
+            }
         if (last) {
-            __syncthreads();
+            write_back();
             return;
         }
         T rho2 = (T)bsum1024(v, sh, par);
-        if (fabs(rho2) < 10 * eps_of<T>()) return;
+        if (fabs(rho2) < 10 * eps_of<T>()) {
+            write_back();
+            return;
+        }
         T beta = rho2 / rho;
-        SLOOP(l, { l.e[c] = beta * l.e[c] + l.z[c]; })
+#pragma unroll
+        for (int k = 0; k < TAIL_KC; ++k)
+            if (on[k]) {
+                e[k] = beta * e[k] + z[k];
+                l.e[cc[k]] = e[k];
+            }
+        __syncthreads();
         rho = rho2;
     }
+    write_back();  // it == 0
 }
 // LDS bytes the levels l0..nlev-1 need
 template <class T> inline size_t tail_lds_bytes(const Grid *gs, int l0, int nlev) {
