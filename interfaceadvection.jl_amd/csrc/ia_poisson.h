@@ -1501,6 +1501,81 @@ template <int D, class T> __device__ void stail_pcg(const SLev<T> &l, int it, do
     }
     write_back();  // it == 0
 }
+// pcg!(l; it) on a level of at most 64 cells, by wavefront 0 alone, one
+// cell per lane.  Its sums are the wavefront's butterfly sums plus 0.0: in
+// stail_pcg the other wavefronts' partials would all be 0, so the values are
+// the same.  The lanes exchange ϵ through LDS with no workgroup barrier; a
+// wavefront's LDS accesses complete in program order, and the fences keep
+// the compiler from moving them across one another.
+__device__ inline void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <int D, class T> __device__ void stail_pcg_wave(const SLev<T> &l, int it) {
+    const int c = threadIdx.x;
+    const bool on = c < l.ncell;
+    int q[3] = {0, 0, 0};
+    T x = 0, r = 0, z = 0, e = 0, iD = 0;
+    double v = 0;
+    if (on) {
+        scoord(l.n, c, q);
+        x = l.x[c];
+        r = l.r[c];
+        iD = l.iD[c];
+        T zz = r * iD;
+        z = zz;
+        e = zz;
+        l.e[c] = zz;
+        v += (double)r * (double)zz;
+    }
+    T rho = (T)(0.0 + wave_sum(v));
+    if (fabs(rho) < 10 * eps_of<T>()) return;  // x and r unchanged
+    wave_lds_order();
+    for (int i = 1; i <= it; ++i) {
+        v = 0;
+        if (on) {
+            T zz = smult<D>(l, l.e, c, q);
+            z = zz;
+            v += (double)zz * (double)e;
+        }
+        T alpha = rho / (T)(0.0 + wave_sum(v));
+        bool last = i == it || fabs((double)alpha) < 1e-2;
+        v = 0;
+        if (on) {
+            x += alpha * e;
+            T rr = r - alpha * z;
+            r = rr;
+            T z2 = rr * iD;
+            z = z2;
+            v += (double)rr * (double)z2;
+        }
+        if (last) break;
+        T rho2 = (T)(0.0 + wave_sum(v));
+        if (fabs(rho2) < 10 * eps_of<T>()) break;
+        T beta = rho2 / rho;
+        wave_lds_order();
+        if (on) {
+            e = beta * e + z;
+            l.e[c] = e;
+        }
+        wave_lds_order();
+        rho = rho2;
+    }
+    if (on) {
+        l.x[c] = x;
+        l.r[c] = r;
+    }
+}
+// pcg! on a tail level; returns past a workgroup barrier
+template <int D, class T> __device__ void stail_solve(const SLev<T> &l, int it, double *sh, int &par) {
+    if (l.ncell <= 64) {
+        if (threadIdx.x < 64) stail_pcg_wave<D>(l, it);
+        __syncthreads();
+    } else {
+        stail_pcg<D>(l, it, sh, par);
+    }
+}
 // LDS bytes the levels l0..nlev-1 need
 template <class T> inline size_t tail_lds_bytes(const Grid *gs, int l0, int nlev) {
     size_t b = 0;
@@ -1570,7 +1645,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
     // ascend: pcg!(l+1), prolongate!, increment!(l)
     for (int l = last - 1; l >= a.l0; --l) {
         SLev<T> f = slev(a, l, smem), cs = slev(a, l + 1, smem);
-        stail_pcg<D>(cs, a.cit[l + 1], sh, par);
+        stail_solve<D>(cs, a.cit[l + 1], sh, par);
         SLOOP(f, {
             int k0 = q[0] / 2, k1 = f.D > 1 ? q[1] / 2 : 0, k2 = f.D > 2 ? q[2] / 2 : 0;
             f.e[c] = cs.x[k0 + cs.n[0] * (k1 + cs.n[1] * k2)];
@@ -1578,7 +1653,7 @@ template <int D, class T> __device__ void tail_run(const TailArgs<T> &a, T *smem
         stail_increment<D>(f);
     }
     SLev<T> s = slev(a, a.l0, smem);
-    stail_pcg<D>(s, a.cit[a.l0], sh, par);
+    stail_solve<D>(s, a.cit[a.l0], sh, par);
     // the top level's solution back to global memory (interior)
     const TailLev<T> &G = a.lv[a.l0];
     SLOOP(s, {
