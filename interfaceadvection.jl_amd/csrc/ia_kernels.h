@@ -9,9 +9,28 @@ constexpr int NT = TX * TY;  // threads per block (256 = 4 wave64)
 // ---------------------------------------------------------------------------
 // block reductions (deterministic: fixed shuffle tree, fixed LDS order)
 // ---------------------------------------------------------------------------
+// a double moved across lanes by a DPP pattern (both halves; every source
+// lane is inside the row, so the bound control never applies)
+template <int CTRL> __device__ inline double dpp_d(double v) {
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double lane_d(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// sum over the 64 lanes of a fully active wavefront, the same value in every
+// lane: pairwise within each 16-lane row (lane pairs l^1, l^2 by quad
+// permutes, then the half-row and row mirrors), each pair added in the same
+// two operands by both of its lanes; then the four row sums in row order.
+// DPP and readlane instead of LDS permutes keep it off the LDS pipeline.
 __device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    return ((lane_d(v, 0) + lane_d(v, 16)) + lane_d(v, 32)) + lane_d(v, 48);
 }
 __device__ inline double block_sum(double v) {
     __shared__ double sh[NT / 64];
