@@ -748,6 +748,38 @@ __global__ void __launch_bounds__(NT) k_cg1_step(LevV<T> l, Cg1V<T> b, PcgState 
     }
     T *__restrict__ wo = b.wo;
     double g = 0, d = 0;
+    if (l.g.D == 3) {
+        // u' = r'·iD evaluated once per cell into the plane tile (z-neighbours
+        // in registers) instead of at each of its seven stencil uses
+        struct Raw {
+            T r, w, s, d;
+        };
+        struct Pre {
+            L6<T> l;
+            T r, w, s, d, p, x;
+        };
+        tiled7<T, Raw>(
+            l.g, l.t, [&](long K) { return Raw{r[K], w[K], first ? T(0) : s[K], iD[K]}; },
+            [&](const Raw &q) { return (q.r - alpha * (first ? q.w : q.w + beta * q.s)) * q.d; },
+            [&](long I) { return Pre{load_L6<T>(l.g, I, l.L), r[I], w[I], first ? T(0) : s[I], iD[I], pd[I], x[I]}; },
+            [&](const int *, long I, const Pre &pv, T c, T xm, T xp, T ym, T yp, T zm, T zp) {
+                T u = pv.r * pv.d;
+                T pp = first ? u : u + beta * pv.p;
+                T ss = first ? pv.w : pv.w + beta * pv.s;
+                T rr = pv.r - alpha * ss;
+                T ww = mult7<T>(pv.l, c, xm, xp, ym, yp, zm, zp);
+                pd[I] = pp;
+                so[I] = ss;
+                x[I] = pv.x + alpha * pp;
+                ro[I] = rr;
+                wo[I] = ww;
+                g += (double)rr * (double)c;
+                d += (double)c * (double)ww;
+            });
+        block_partial(g, pout, 2, 0);
+        block_partial(d, pout, 2, 1);
+        return;
+    }
     DISPATCH_D(l.g, {
         col_pipe(l.g, l.t,
                  [&](const int *qq, long I) { return ld7<D>(l.g, qq, I, l.L, [&](long K) { return rK(K) * iD[K]; }); },
