@@ -511,6 +511,18 @@ __global__ void __launch_bounds__(NT) k_pcg_init(LevV<T> l, double *partA) {
 template <int D, class T> __device__ inline void pcg_Ae_d(const LevV<T> &l, double &v) {
     const T *__restrict__ e = l.e;
     T *__restrict__ z = l.z;
+    if (D == 3 && l.t.zb > 1) {
+        // column tilings: ϵ through an LDS plane tile (95.8 -> 81.5 µs at 256³)
+        tiled7<T, T>(
+            l.g, l.t, [&](long K) { return e[K]; }, [&](T q) { return q; },
+            [&](long I) { return load_L6<T>(l.g, I, l.L); },
+            [&](const int *, long I, const L6<T> &lv, T c, T xm, T xp, T ym, T yp, T zm, T zp) {
+                T zz = mult7<T>(lv, c, xm, xp, ym, yp, zm, zp);
+                z[I] = zz;
+                v += (double)zz * (double)c;
+            });
+        return;
+    }
     col_pipe(l.g, l.t, [&](const int *p, long I) { return ld7<D>(l.g, p, I, l.L, [&](long K) { return e[K]; }); },
              [&](const int *, long I, const S7<D, T> &s) {
                  T zz = mul7<D>(s);
