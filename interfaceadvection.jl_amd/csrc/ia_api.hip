@@ -476,10 +476,18 @@ template <class T> struct Impl {
         }
         if (pc >= 0) prof_end(c, pc, it);
     }
-    static void vcycle(ia_ctx *c, int l, const int *cit, bool pre_done = false, bool then_pcg = false) {
+    // e_done: Jacobi!'s ϵ = r·iD on level l was written by the restrict! that
+    // produced its r, so only increment! remains.
+    static void vcycle(ia_ctx *c, int l, const int *cit, bool pre_done = false, bool then_pcg = false,
+                       bool e_done = false) {
         Level &fl = c->lev[l], &co = c->lev[l + 1];
-        if (!pre_done) jacobi(c, fl);
-        hipLaunchKernelGGL(k_mg_restrict<T>, cb(co), dim3(NT), 0, c->s, view(co), fl.g, P<T>(fl.r));
+        if (!pre_done) {
+            if (e_done) increment(c, fl);
+            else jacobi(c, fl);
+        }
+        bool rec = l + 2 < c->nlev, co_jacobi = l + 1 < c->tail_level && rec;
+        hipLaunchKernelGGL(k_mg_restrict<T>, cb(co), dim3(NT), 0, c->s, view(co), fl.g, P<T>(fl.r),
+                           co_jacobi ? 1 : 0);
         if (l + 1 >= c->tail_level) {
             TailArgs<T> a;
             for (int q = 0; q < c->nlev && q < MAXLEV; ++q) {
@@ -500,8 +508,7 @@ template <class T> struct Impl {
             a.halt = cur_halt;
             hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(NT1), c->tail_lds, c->s, a);
         } else {
-            bool rec = l + 2 < c->nlev;
-            if (rec) vcycle(c, l + 1, cit, false, true);
+            if (rec) vcycle(c, l + 1, cit, false, true, true);
             if (cit == c->cit_fcg) pcg1(c, co, cit[l + 1]);
             else pcg(c, co, cit[l + 1], rec);
         }

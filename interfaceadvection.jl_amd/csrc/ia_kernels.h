@@ -71,6 +71,70 @@ __device__ inline void fill_combine(FillPart &a, const FillPart &b) {
     if (b.vmin < a.vmin || (b.vmin == a.vmin && b.imin < a.imin)) { a.vmin = b.vmin; a.imin = b.imin; }
     a.nan |= b.nan;
 }
+// a wavefront's max / min of a double and the smallest index holding it, by
+// the DPP pattern of wave_sum (order-independent, so equal to block_fill's)
+template <int CTRL> __device__ inline long dpp_l(long v) {
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v & 0xffffffffL), CTRL, 0xF, 0xF, true);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, 0xF, 0xF, true);
+    return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline long lane_l(long v, int l) {
+    unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffL), l);
+    unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+    return (long)(((unsigned long)hi << 32) | lo);
+}
+__device__ inline double wave_max_d(double v) {
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x141>(v));
+    v = fmax(v, dpp_d<0x140>(v));
+    return fmax(fmax(fmax(lane_d(v, 0), lane_d(v, 16)), lane_d(v, 32)), lane_d(v, 48));
+}
+__device__ inline double wave_min_d(double v) {
+    v = fmin(v, dpp_d<0xB1>(v));
+    v = fmin(v, dpp_d<0x4E>(v));
+    v = fmin(v, dpp_d<0x141>(v));
+    v = fmin(v, dpp_d<0x140>(v));
+    return fmin(fmin(fmin(lane_d(v, 0), lane_d(v, 16)), lane_d(v, 32)), lane_d(v, 48));
+}
+__device__ inline long wave_min_l(long v) {
+    long t;
+    t = dpp_l<0xB1>(v); v = t < v ? t : v;
+    t = dpp_l<0x4E>(v); v = t < v ? t : v;
+    t = dpp_l<0x141>(v); v = t < v ? t : v;
+    t = dpp_l<0x140>(v); v = t < v ? t : v;
+    long a = lane_l(v, 0), b = lane_l(v, 16), c = lane_l(v, 32), d = lane_l(v, 48);
+    a = b < a ? b : a;
+    a = c < a ? c : a;
+    return d < a ? d : a;
+}
+// block_fill of one value per thread (ok: the thread has a cell and its value
+// is not NaN; nan: its value is NaN), equal to building each thread's
+// FillPart and calling block_fill: the wavefront reduces by DPP, then thread
+// 0 combines the wavefronts' parts.  Valid in thread 0.
+__device__ inline FillPart block_fill_cells(double v, long I, bool ok, bool nan) {
+    __shared__ FillPart shw[NT / 64];
+    double wmx = wave_max_d(ok ? v : -1e300), wmn = wave_min_d(ok ? v : 1e300);
+    long imx = wave_min_l(ok && v == wmx ? I : (1L << 60));
+    long imn = wave_min_l(ok && v == wmn ? I : (1L << 60));
+    unsigned long long anynan = __ballot(nan);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        FillPart p;
+        p.vmax = wmx;
+        p.vmin = wmn;
+        p.imax = imx;
+        p.imin = imn;
+        p.nan = anynan != 0;
+        p.pad = 0;
+        shw[w] = p;
+    }
+    __syncthreads();
+    FillPart r = shw[0];
+    if (threadIdx.x == 0)
+        for (int q = 1; q < NT / 64; ++q) fill_combine(r, shw[q]);
+    return r;
+}
 __device__ inline FillPart block_fill(FillPart v) {
     __shared__ FillPart sh[NT];
     sh[threadIdx.x] = v;
@@ -199,11 +263,9 @@ __global__ void __launch_bounds__(NT) k_vof_update(SweepArgs<T> a, const T *__re
     const Grid &g = a.g;
     int p[3];
     bool act = thread_cell(g, a.t, p);
-    FillPart fp;
-    fp.vmax = -1e300;
-    fp.vmin = 1e300;
-    fp.imax = fp.imin = 1L << 60;
-    fp.nan = 0;
+    double fv = 0;
+    long fi = 0;
+    bool fok = false, fnan = false;
     if (act) {
         int d = a.d;
         long I = lin(g, p), sd = g.st[d];
@@ -211,17 +273,15 @@ __global__ void __launch_bounds__(NT) k_vof_update(SweepArgs<T> a, const T *__re
         T ff0 = face[I], ff1 = face[I + sd];
         T dud = u1[I + sd] - u1[I], du0d = u2[I + sd] - u2[I];
         T fnew = a.fin[I] + ((ff0 - ff1) + (((T)a.cb[I] * (dud + du0d)) * a.dt) / 2);
-        if (fnew != fnew) fp.nan = 1;
-        else {
-            fp.vmax = (double)fnew;
-            fp.vmin = (double)fnew;
-            fp.imax = fp.imin = I;
-        }
+        fnan = fnew != fnew;
+        fok = !fnan;
+        fv = (double)fnew;
+        fi = I;
         T fc = fnew < a.ctol ? T(0) : (fnew > 1 - a.ctol ? T(1) : fnew);
         write_images<T>(g, a.fout, p, fc, 0, 0, T(0));
         if (a.fmid) write_images<T>(g, a.fmid, p, (fc + a.fn[I]) / 2, 0, 0, T(0));
     }
-    FillPart r = block_fill(fp);
+    FillPart r = block_fill_cells(fv, fi, fok, fnan);
     if (threadIdx.x == 0) a.part[blockIdx.x] = r;
 }
 

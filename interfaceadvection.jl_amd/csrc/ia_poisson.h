@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(NT) k_mg_increment(LevV<T> l) {
 }
 // restrict!: coarse r = Σ children; coarse x = 0
 template <class T>
-__global__ void __launch_bounds__(NT) k_mg_restrict(LevV<T> c, Grid gf, const T *rf) {
+__global__ void __launch_bounds__(NT) k_mg_restrict(LevV<T> c, Grid gf, const T *rf, int with_e) {
     HALT_IF(c)
     COL_LOOP(c, {
         int lo[3], hi[3];
@@ -426,6 +426,7 @@ __global__ void __launch_bounds__(NT) k_mg_restrict(LevV<T> c, Grid gf, const T 
                 for (int c0 = lo[0]; c0 <= hi[0]; ++c0) s += rf[c0 + gf.st[1] * c1 + gf.st[2] * c2];
         c.r[I] = s;
         c.x[I] = T(0);
+        if (with_e) c.e[I] = s * c.iD[I];  // the Jacobi! step that follows on this level
     })
 }
 // prolongate!: fine ϵ = coarse x[down(I)]
