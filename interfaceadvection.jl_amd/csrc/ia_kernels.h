@@ -181,6 +181,23 @@ __global__ void __launch_bounds__(NT) k_prepare(Grid g, Tiling t, const T *__res
 // cell pass makes the conservative update with the c̄ dilation term, the fill
 // report, cleanWisp! and BCf! (advection.jl:80-137, normalEstimation.jl:10-28)
 // ---------------------------------------------------------------------------
+// the coordinates of flat index K of the ghost-inclusive grid; 32-bit
+// division whenever the grid has fewer than 2^31 entries (64-bit integer
+// division costs several times more)
+__device__ inline void ghost_coord(const Grid &g, long K, int q[3]) {
+    if (g.ns < (1L << 31)) {
+        unsigned k = (unsigned)K, n0 = (unsigned)g.Ng[0], n1 = (unsigned)g.Ng[1];
+        unsigned r = k / n0;
+        q[0] = (int)(k - r * n0);
+        q[1] = (int)(r % n1);
+        q[2] = (int)(r / n1);
+    } else {
+        q[0] = (int)(K % g.Ng[0]);
+        long r = K / g.Ng[0];
+        q[1] = (int)(r % g.Ng[1]);
+        q[2] = (int)(r / g.Ng[1]);
+    }
+}
 template <class T> struct SweepArgs {
     Grid g;
     Tiling t;
@@ -382,10 +399,7 @@ __global__ void __launch_bounds__(NT) k_ustar(MomArgs<T> a) {
     long K = (long)blockIdx.x * NT + threadIdx.x;
     if (K >= g.ns) return;
     int q[3];
-    q[0] = (int)(K % g.Ng[0]);
-    long r = K / g.Ng[0];
-    q[1] = (int)(r % g.Ng[1]);
-    q[2] = (int)(r / g.Ng[1]);
+    ghost_coord(g, K, q);
     for (int i = 0; i < g.D; ++i) a.us[g.ns * i + K] = mom_ustar(a, i, q);
 }
 // the sweep with the axis d and the dimension as compile-time constants
@@ -529,10 +543,7 @@ __global__ void __launch_bounds__(NT) k_fface(ViscArgs<T> a, T *ff) {
     long K = (long)blockIdx.x * NT + threadIdx.x;
     if (K >= g.ns) return;
     int q[3];
-    q[0] = (int)(K % g.Ng[0]);
-    long r = K / g.Ng[0];
-    q[1] = (int)(r % g.Ng[1]);
-    q[2] = (int)(r / g.Ng[1]);
+    ghost_coord(g, K, q);
     for (int c = 0; c < g.D; ++c) ff[g.ns * c + K] = visc_ff(a, c, q);
 }
 // viscF(i, j, Q) (flow.jl:144, getμ VOFutil.jl:186-191) on loaded values:
