@@ -487,10 +487,18 @@ __global__ void __launch_bounds__(NT) k_mom_face(MomArgs<T> a, T *__restrict__ f
     bool top = DIR != 0 && p[DIR] == g.Ng[DIR] - 2;
     int pu[3] = {p[0], p[1], p[2]};
     pu[DIR] += 1;
+    // every component's face value before any store, so that the loads of
+    // the components overlap (a store could alias them)
+    T fv[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-        face[g.ns * i + I] = mom_face<DIR>(a, i, p);
-        if (top) face[g.ns * i + I + sd] = mom_face<DIR>(a, i, pu);
+    for (int i = 0; i < D; ++i) fv[i] = mom_face<DIR>(a, i, p);
+#pragma unroll
+    for (int i = 0; i < D; ++i) face[g.ns * i + I] = fv[i];
+    if (top) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) fv[i] = mom_face<DIR>(a, i, pu);
+#pragma unroll
+        for (int i = 0; i < D; ++i) face[g.ns * i + I + sd] = fv[i];
     }
 }
 // cell pass: the two-pass difference of the face fluxes, the uOld dilation
@@ -502,6 +510,7 @@ __global__ void __launch_bounds__(NT) k_mom_update(MomArgs<T> a, const T *__rest
     if (!thread_cell(g, a.t, p)) return;
     long I = lin(g, p), sd = g.st[DIR];
     T dl0 = mom_dila<DIR>(a, p);
+    T rn[D];  // every component before any store (a store could alias the loads)
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         T flo = face[g.ns * i + I], fup = face[g.ns * i + I + sd];
@@ -510,8 +519,10 @@ __global__ void __launch_bounds__(NT) k_mom_update(MomArgs<T> a, const T *__rest
         T r = (T(0) + flo) - fup;
         r += a.uold[g.ns * i + I] * ((dl0 + mom_dila<DIR>(a, pm)) / 2);
         long o = g.ns * i + I;
-        a.ru_out[o] = a.ru_in[o] + a.dt * r;
+        rn[i] = a.ru_in[o] + a.dt * r;
     }
+#pragma unroll
+    for (int i = 0; i < D; ++i) a.ru_out[g.ns * i + I] = rn[i];
 }
 // ---------------------------------------------------------------------------
 // viscous stress (flow.jl:113-153) fused with the RK update (flow.jl:244-252)
@@ -822,6 +833,7 @@ __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
     // another's addresses, as no store intervenes
     T Fv[D];
     visc_force_all<D>(a.v, p, Fv);
+    T un[D], rh[D];  // every component before any store (a store could alias the loads)
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         long o = g.ns * i + I;
@@ -830,10 +842,14 @@ __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
         T ru0 = a.u0[o] * linp((a.fcur[I] + a.fcur[I - g.st[i]]) / 2, a.v.lr);
         T run = ((ac * ru0 + a.ru[o]) + F * a.dt) * wT;
         T rho = linp((a.fupd[I] + a.fupd[I - g.st[i]]) / 2, a.v.lr);
-        T un = run / rho;
-        un = un + (a.dt * wT) * a.g[i];
-        write_images<T>(g, a.uout + g.ns * i, p, un, 1, i, a.v.ubc[i]);
-        write_images<T>(g, a.L + g.ns * i, p, T(1) / rho, 1, i, T(0));
+        T u1 = run / rho;
+        un[i] = u1 + (a.dt * wT) * a.g[i];
+        rh[i] = rho;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        write_images<T>(g, a.uout + g.ns * i, p, un[i], 1, i, a.v.ubc[i]);
+        write_images<T>(g, a.L + g.ns * i, p, T(1) / rh[i], 1, i, T(0));
     }
 }
 // viscSurfTenρu! alone (stage entry point): Flow.f = visc
