@@ -435,6 +435,36 @@ def test_nan_is_reported():
     g.close()
 
 
+@pytest.mark.parametrize("D", [2, 3])
+def test_fill_report_extremes_and_ties(D):
+    """reportFillError (advection.jl:145-189): the sweep reports the largest and
+    smallest f before cleanWisp! and, on ties, the first cell in memory order.
+    Still fluid, so the sweep leaves f as it was; two cells share the maximum
+    and two the minimum, each pair straddling wavefronts and blocks."""
+    say("fill report", f"{D}D")
+    case = bubble2d(32, np.float64) if D == 2 else bubbles3d(16, np.float64)
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])
+    g.flow.u = np.zeros_like(g.flow.u)
+    g.flow.u0 = np.zeros_like(g.flow.u0)
+    f = g.intf.f
+    # interior cells, the second of each pair first in memory
+    hi = [(3, 7), (20, 2)] if D == 2 else [(3, 7, 5), (12, 2, 3)]  # both 1.5
+    lo = [(30, 30), (9, 4)] if D == 2 else [(14, 14, 3), (9, 4, 12)]  # both -0.25
+    for c in hi:
+        f[c] = 1.5
+    for c in lo:
+        f[c] = -0.25
+    g.intf.f = f
+    flat = lambda c: int(np.ravel_multi_index(c, f.shape, order="F"))
+    rc = g._call("ia_vof_sweep", 1, 0.5, None)
+    assert rc == 3
+    msg = g._lib.ia_last_error(g._ctx).decode()
+    assert f"max 1.5 @ {min(flat(c) for c in hi)}," in msg, msg
+    assert f"min -0.25 @ {min(flat(c) for c in lo)}" in msg, msg
+    assert min(flat(c) for c in hi) == flat(hi[1]) and len({flat(c) // 64 for c in hi}) == 2
+    g.close()
+
+
 # --------------------------------------------------------------------------- plug-ins
 def _pair_plugin(case, scheme="WH", lim="Koren"):
     """GPU and oracle with one plug-in; the pressure solves run to round-off
