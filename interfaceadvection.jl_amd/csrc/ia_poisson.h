@@ -58,16 +58,20 @@ struct FcgState {
 // every block of a consumer sums the producer's per-block partials itself
 // (fixed order: strided per-thread sums, then the block tree), so a
 // reduction costs neither an atomic nor an extra launch
+// every thread sums the wavefronts' sums in wavefront order (block_sum's
+// order, so the same value) instead of waiting for thread 0 to broadcast:
+// two barriers instead of four
 __device__ inline double block_total(const double *part, long n, int K = 1, int k = 0) {
-    __shared__ double tot;
+    __shared__ double sht[NT / 64];
     double v = 0;
     for (long q = threadIdx.x; q < n; q += NT) v += part[K * q + k];
-    double s = block_sum(v);
-    if (threadIdx.x == 0) tot = s;
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sht[threadIdx.x >> 6] = v;
     __syncthreads();
-    double r = tot;
-    __syncthreads();
-    return r;
+    double s = 0;
+    for (int q = 0; q < NT / 64; ++q) s += sht[q];
+    __syncthreads();  // before another block_total of this kernel rewrites sht
+    return s;
 }
 __device__ inline void block_partial(double v, double *part, int K = 1, int k = 0) {
     double s = block_sum(v);
@@ -817,12 +821,14 @@ __global__ void __launch_bounds__(NT) k_cg1_step(LevV<T> l, Cg1V<T> b, PcgState 
 
 // two block sums in one pass (same per-thread order and tree as two calls of
 // block_sum / block_total, so the same values)
-__device__ inline void block_sum2(double &a, double &b) {  // valid in thread 0
+// lead: the barrier that protects sh2 from an earlier block_sum2 of the
+// same kernel (not needed on its first use)
+__device__ inline void block_sum2(double &a, double &b, bool lead = true) {  // valid in thread 0
     __shared__ double sh2[2][NT / 64];
     a = wave_sum(a);
     b = wave_sum(b);
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
+    if (lead) __syncthreads();
     if (lane == 0) {
         sh2[0][w] = a;
         sh2[1][w] = b;
@@ -904,17 +910,21 @@ __global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState
     T alpha = 0, beta = 0;
     bool reduce = !st.done && !st.stop;
     if (reduce) {
-        // block_total2, with the per-thread sums already loaded
-        __shared__ double tot2[2];
-        block_sum2(va, vb);
-        if (threadIdx.x == 0) {
-            tot2[0] = va;
-            tot2[1] = vb;
+        // block_total2, with the per-thread sums already loaded; every
+        // thread sums the wavefronts' sums in order (one barrier)
+        __shared__ double shs[2][NT / 64];
+        va = wave_sum(va);
+        vb = wave_sum(vb);
+        if ((threadIdx.x & 63) == 0) {
+            shs[0][threadIdx.x >> 6] = va;
+            shs[1][threadIdx.x >> 6] = vb;
         }
         __syncthreads();
-        T gam = (T)tot2[0];
-        T del = (T)tot2[1];
-        __syncthreads();
+        double sa = 0, sb = 0;
+        for (int w = 0; w < NT / 64; ++w) sa += shs[0][w];
+        for (int w = 0; w < NT / 64; ++w) sb += shs[1][w];
+        T gam = (T)sa;
+        T del = (T)sb;
         if (fabs(gam) < 10 * eps_of<T>()) {
             st.done = 1;
         } else {
@@ -964,7 +974,7 @@ __global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState
         }
     }
     if (st.stop) return;
-    block_sum2(gd, dd);
+    block_sum2(gd, dd, false);  // sh2's first use in this kernel
     if (threadIdx.x == 0) {
         pout[2 * blockIdx.x] = gd;
         pout[2 * blockIdx.x + 1] = dd;
