@@ -18,7 +18,10 @@ Prints one JSON line (see the driver contract): value = whole-job
 Mcell-steps/s, roofline of the dominant kernel (algorithmic bytes per launch
 / average launch time from HIP events recorded around that kernel during the
 timed region), and the CPU oracle timed on the host cores over a bounded
-sample of the same workload.
+sample of the same workload.  The other kernel classes are timed by events
+over the warm-up steps (the dominant class is the one with the largest total
+there); recording every class inside the timed steps costs about 3% of a step
+(`--events all` does so).
 """
 import argparse
 import importlib
@@ -182,6 +185,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain / refined / learned solver comparison")
     ap.add_argument("--no-events", action="store_true", help="time the steps without the per-launch HIP events")
+    ap.add_argument("--events", choices=["dominant", "all"], default="dominant",
+                    help="classes timed by HIP events inside the timed steps: the dominant one (the others are timed "
+                         "over the warm-up steps), or all of them")
     ap.add_argument("--compare-warm", type=int, default=8)
     ap.add_argument("--compare-steps", type=int, default=3)
     args = ap.parse_args()
@@ -202,10 +208,34 @@ def main():
             sim.learned_init(seed=0)
     print(f"[bench] setup {N}^3 with {len(bl)} bubbles in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
+    classes = [] if args.no_events else (PROFILED if args.profile == "auto" else [args.profile])
+    cells = N ** 3
+
+    def read_classes(names):
+        out = {}
+        for k in names:
+            nl, ms = ia.C.c_int(), ia.C.c_double()
+            sim._call("ia_profile_read", ia.PROFILE[k], ia.C.byref(nl), ia.C.byref(ms))
+            sim._call("ia_profile_enable", ia.PROFILE[k], 0)
+            if nl.value:
+                avg = ms.value / nl.value
+                kc = (N >> LEVEL.get(k, 0)) ** 3
+                out[k] = dict(launches=nl.value, total_ms=round(ms.value, 3), avg_ms=round(avg, 4),
+                              GBps=round(BYTES_PER_CELL[k] * kc / (avg * 1e-3) / 1e9, 1), cells=kc)
+        return out
+
+    # Every class is timed over the warm-up steps; inside the timed steps only
+    # the dominant one (largest total over the warm-up) keeps its events, as
+    # the events of every class cost about 3% of the step.
+    warm_classes = classes if (args.events == "dominant" and args.warmup > 0) else []
+    for k in warm_classes:
+        sim._call("ia_profile_enable", ia.PROFILE[k], 1)
     for _ in range(args.warmup):
         ia.sim_step_(sim)
     sim._call("ia_synchronize")
-    classes = [] if args.no_events else (PROFILED if args.profile == "auto" else [args.profile])
+    warm = read_classes(warm_classes)
+    if warm:
+        classes = [max(warm, key=lambda k: warm[k]["total_ms"])]
     for k in classes:
         sim._call("ia_profile_enable", ia.PROFILE[k], 1)
     iters, solve_ms, rel = [], [], []
@@ -218,17 +248,7 @@ def main():
             rel.append(st.solve[q].rel)
     sim._call("ia_synchronize")
     t = time.perf_counter() - t0
-    cells = N ** 3
-    timed = {}
-    for k in classes:
-        nl, ms = ia.C.c_int(), ia.C.c_double()
-        sim._call("ia_profile_read", ia.PROFILE[k], ia.C.byref(nl), ia.C.byref(ms))
-        sim._call("ia_profile_enable", ia.PROFILE[k], 0)
-        if nl.value:
-            avg = ms.value / nl.value
-            kc = (N >> LEVEL.get(k, 0)) ** 3
-            timed[k] = dict(launches=nl.value, total_ms=round(ms.value, 3), avg_ms=round(avg, 4),
-                            GBps=round(BYTES_PER_CELL[k] * kc / (avg * 1e-3) / 1e9, 1), cells=kc)
+    timed = read_classes(classes)
     dom = max(timed, key=lambda k: timed[k]["total_ms"]) if timed else None
     print(f"[bench] {args.steps} steps in {t:.3f} s; V-cycles per solve {iters}", file=sys.stderr, flush=True)
     value = cells * args.steps / t / 1e6
@@ -252,7 +272,9 @@ def main():
                    "cg_iterations_per_solve_mean": float(np.mean(iters)),
                    "poisson_time_to_1e-6_ms": round(float(np.mean(solve_ms)), 3),
                    "poisson_rel_residual_max": float(np.max(rel)) if rel else None},
-        "kernels": timed,
+        "kernels": timed if not warm else warm,
+        "kernels_timed_over": "the timed steps" if not warm else f"the {args.warmup} warm-up steps "
+                              f"(the roofline kernel: the timed steps)",
     }
     if dom:
         avg_ms = timed[dom]["avg_ms"]
