@@ -1073,6 +1073,29 @@ __global__ void __launch_bounds__(NT) k_fcg_center_z(FcgV<T> f, const double *pa
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_resid(FcgV<T> f, double *part) {
     double v = 0;
+    if (f.g.D == 3 && f.t.zb > 1) {
+        // column tilings: x through an LDS plane tile (mult64's operations)
+        const double *__restrict__ x = f.x;
+        const double *__restrict__ z = f.z;
+        const T *__restrict__ iD = f.iD;
+        double *__restrict__ r = f.r;
+        struct Pre {
+            L6<double> l;
+            double z;
+            T d;
+        };
+        tiled7<double, double>(
+            f.g, f.t, [&](long K) { return x[K]; }, [&](double q) { return q; },
+            [&](long I) { return Pre{load_L6<double>(f.g, I, f.L), z[I], iD[I]}; },
+            [&](const int *, long I, const Pre &pv, double c, double xm, double xp, double ym, double yp, double zm,
+                double zp) {
+                double rr = pv.d == T(0) ? 0.0 : pv.z - mult7<double>(pv.l, c, xm, xp, ym, yp, zm, zp);
+                r[I] = rr;
+                v += rr;
+            });
+        block_partial(v, part);
+        return;
+    }
     COL_LOOP(f, {
         double rr = f.iD[I] == T(0) ? 0.0 : f.z[I] - mult64(f.g, p, I, f.L, f.x);
         f.r[I] = rr;
@@ -1333,6 +1356,31 @@ template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, double dtw, T ubc0, T ubc1, T ubc2) {
     const Grid &g = f.g;
     T ub[3] = {ubc0, ubc1, ubc2};
+    if (g.D == 3 && f.t.zb > 1) {
+        // column tilings: x through an LDS plane tile (its low neighbours are
+        // the periodic images xper reads)
+        const double *__restrict__ x = f.x;
+        struct Pre {
+            T l0, l1, l2, u0, u1, u2;
+        };
+        tiled7<double, double>(
+            g, f.t, [&](long K) { return x[K]; }, [&](double q) { return q; },
+            [&](long I) {
+                return Pre{f.L[I], f.L[g.ns + I], f.L[2 * g.ns + I], u[I], u[g.ns + I], u[2 * g.ns + I]};
+            },
+            [&](const int *pc, long I, const Pre &pv, double c, double xm, double, double ym, double, double zm,
+                double) {
+                int p[3] = {pc[0], pc[1], pc[2]};
+                T un0 = (T)((double)pv.u0 - (double)pv.l0 * (c - xm));
+                T un1 = (T)((double)pv.u1 - (double)pv.l1 * (c - ym));
+                T un2 = (T)((double)pv.u2 - (double)pv.l2 * (c - zm));
+                write_images<T>(g, u, p, un0, 1, 0, ub[0]);
+                write_images<T>(g, u + g.ns, p, un1, 1, 1, ub[1]);
+                write_images<T>(g, u + 2 * g.ns, p, un2, 1, 2, ub[2]);
+                pT[I] = (T)(c / dtw);
+            });
+        return;
+    }
     COL_LOOP(f, {
         for (int i = 0; i < g.D; ++i) {
             int q[3] = {p[0], p[1], p[2]};
