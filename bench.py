@@ -57,6 +57,7 @@ BYTES_PER_CELL = {
     "load_inc": 8 + 4 + 12 + 8,                      # r64, iD, L -> r, x
     "prolong_inc": 4 + 4 + 12 + 4 + 0.5 + 12,        # r, x, L, iD, coarse x -> r, x, ϵ
     "pcg_last": 4 + 4 + 8 + 8 + 4,                   # x, ϵ, r64, q64 -> x
+    "cheb2_dots": 4 + 12 + 4 + 4 + 4 + 8 + 8 + 4,    # ϵ, L, r, x, iD, r64, q64 -> x
     "fcg_Ap": 4 + 8 + 12 + 16,                       # w, p, L -> p', q
     "fcg_update": 32 + 16,                           # p', q, x64, r64 -> x64, r64
     "redist": (12 + 16 + 16) / 3,                    # ϕ, ϕini (+ ϕ⁰ in stages 2, 3) -> ϕ', per stage
@@ -68,7 +69,7 @@ BYTES_PER_CELL = {
 LEVEL = {"cg1_l1": 1, "cg1_l2": 2, "cg1_l3": 3}
 # kernel classes timed with HIP events in the timed region; the one with the
 # largest total time is reported as the dominant kernel
-PROFILED = ["load_inc", "prolong_inc", "pcg_Ae", "pcg_last", "fcg_Ap", "fcg_update", "vof", "momentum",
+PROFILED = ["load_inc", "prolong_inc", "cheb2_dots", "fcg_Ap", "fcg_update", "vof", "momentum",
             "visc_update", "cg1_l1", "cg1_l2", "cg1_l3"]
 
 

@@ -241,7 +241,8 @@ enum ia_prof_class {
     IA_PROF_PROLONG_INC = 7,  /* prolongate! + increment! + pcg! ϵ = r·iD, finest level */
     IA_PROF_FCG_AP = 8,       /* FCG p = w + βp, q = A p (FP64) */
     IA_PROF_FCG_UPDATE = 9,   /* FCG x += αp, r -= αq (FP64) */
-    IA_PROF_PCG_LAST = 10,    /* last finest-level pcg! update fused with the FCG dots */
+    IA_PROF_PCG_LAST = 10,    /* the flexible CG's fine smoother fused with its dots: two Chebyshev steps
+                                 (default), or the last finest-level pcg! update (IA_FCG_SMOOTH set) */
     IA_PROF_REDIST = 11,      /* one level-set reinitialisation stage (computeL! + update + BCf!) */
     IA_PROF_CG1_L1 = 12,      /* single-reduction CG step of the FCG preconditioner, coarse level 1 */
     IA_PROF_CG1_L2 = 13,      /* ... level 2 */

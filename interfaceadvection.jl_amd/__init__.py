@@ -24,7 +24,7 @@ FIELD = dict(f=0, f0=1, alpha=2, nhat=3, cbar=5, rhou=6, rhouf=7, u=9, u0=10, fo
 _VECTOR = {"nhat", "rhou", "rhouf", "u", "u0", "force", "mu0"}
 POISSON = {"reference": 0, "refined": 1, "learned": 2, "plain": 3}
 PROFILE = {"vof": 0, "momentum": 1, "visc_update": 2, "pcg_Ae": 3, "pcg_update": 4, "increment": 5,
-           "load_inc": 6, "prolong_inc": 7, "fcg_Ap": 8, "fcg_update": 9, "pcg_last": 10, "redist": 11,
+           "load_inc": 6, "prolong_inc": 7, "fcg_Ap": 8, "fcg_update": 9, "pcg_last": 10, "cheb2_dots": 10, "redist": 11,
            "cg1_l1": 12, "cg1_l2": 13, "cg1_l3": 14}
 STATUS = {0: "OK", 1: "NaN", 2: "divergence explosion", 3: "fill warning", 4: "HIP error", 5: "bad argument"}
 

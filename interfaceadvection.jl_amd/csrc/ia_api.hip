@@ -67,7 +67,8 @@ struct ia_ctx {
     int use_graphs = 0;    // IA_USE_GRAPHS=1: replay each FCG iteration as a HIP graph
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
-    int smooth_it = 2;   // pcg! iterations on the finest level after each preconditioning V-cycle
+    int fine_cheb = 1;   // the finest level's smoother after each preconditioning V-cycle: two Chebyshev steps
+    int smooth_it = 2;   // ... or, with IA_FCG_SMOOTH=n, n pcg! iterations
     // pcg! iterations on each coarse level inside Vcycle!: WaterLily's 6 (REFERENCE,
     // PLAIN), and fcg_coarse_its(level cells) in the flexible CG's preconditioner
     // (IA_FCG_CIT=a,b,... overrides levels 1, 2, ...; the last value repeats)
@@ -621,7 +622,15 @@ template <class T> struct Impl {
         prof_begin(c, IA_PROF_LOAD_INC);
         hipLaunchKernelGGL(k_fcg_load_inc<T>, B, dim3(NT), 0, c->s, f, view(l0), pp(c, rbuf));
         prof_end(c, IA_PROF_LOAD_INC);
-        if (c->smooth_it > 0) {
+        if (c->fine_cheb) {
+            vcycle(c, 0, c->cit_fcg, true, true);
+            double c0, ca, cb;
+            cheb2_coeffs(c0, ca, cb);
+            prof_begin(c, IA_PROF_PCG_LAST);
+            hipLaunchKernelGGL(k_cheb2_dots<T>, B, dim3(NT), 0, c->s, view(l0), f, (T)c0, (T)ca, (T)cb, pp(c, PP_FD),
+                               first);
+            prof_end(c, IA_PROF_PCG_LAST);
+        } else if (c->smooth_it > 0) {
             vcycle(c, 0, c->cit_fcg, true, true);
             pcg(c, l0, c->smooth_it, true, &f, first);
         } else {
@@ -1394,7 +1403,10 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     c->dev = p->device;
     c->es = p->dtype == IA_F64 ? 8 : 4;
     if (const char *gv = getenv("IA_USE_GRAPHS")) c->use_graphs = atoi(gv) != 0;
-    if (const char *sv = getenv("IA_FCG_SMOOTH"); sv && *sv) c->smooth_it = std::max(0, std::min(CIT_MAX, atoi(sv)));
+    if (const char *sv = getenv("IA_FCG_SMOOTH"); sv && *sv) {
+        c->smooth_it = std::max(0, std::min(CIT_MAX, atoi(sv)));
+        c->fine_cheb = 0;
+    }
     for (int l = 0; l < MAXLEV; ++l) {
         long n = 1;
         for (int a = 0; a < p->D; ++a) n *= std::max(1L, (long)(p->N[a] >> l));

@@ -1211,6 +1211,79 @@ __global__ void __launch_bounds__(NT) k_pcg_last_dots(LevV<T> l, FcgV<T> f, PcgS
     block_partial(a, part, 2, 0);
     block_partial(b, part, 2, 1);
 }
+// coefficients of the fine smoother below: Chebyshev on [λmin, λmax] =
+// [0.5, 2] (D⁻¹A has its spectrum in (0, 2]; the smoother damps the upper
+// three quarters), θ = (λmax+λmin)/2, δ = (λmax-λmin)/2, σ = θ/δ, ρ₀ = 1/σ,
+// ρ₁ = 1/(2σ - ρ₀): c0 = 1/θ, ca = ρ₁ρ₀, cb = 2ρ₁/δ (the oracle's mg_cheb2)
+inline void cheb2_coeffs(double &c0, double &ca, double &cb) {
+    const double lmin = 0.5, lmax = 2.0, th = (lmax + lmin) / 2, de = (lmax - lmin) / 2, sg = th / de;
+    const double ro0 = 1 / sg, ro1 = 1 / (2 * sg - ro0);
+    c0 = 1 / th;
+    ca = ro1 * ro0;
+    cb = 2 * ro1 / de;
+}
+// the preconditioner's fine-level smoother: two Chebyshev steps on D⁻¹A for
+// the eigenvalue interval [0.5, 2] (Saad's three-term recurrence; c0 = 1/θ,
+// ca = ρ₁ρ₀, cb = 2ρ₁/δ), fused with the FCG dots.  ϵ = r·iD is on hand from
+// the V-cycle's last prolongate!/increment!, so d₀ = c0·ϵ at every stencil
+// point; r₁ = r - A d₀; d₁ = ca·d₀ + cb·(r₁·iD); x = (x + d₀) + d₁ (the
+// second step's residual update is dead: only x leaves the preconditioner).
+// Then the partials of w·r and w·q with w = s·x.
+template <class T>
+__global__ void __launch_bounds__(NT) k_cheb2_dots(LevV<T> l, FcgV<T> f, T c0, T ca, T cb, double *part, int first) {
+    HALT_IF(l)
+    double s = f.S->sload;
+    double a = 0, b = 0;
+    T *__restrict__ x = l.x;
+    const T *__restrict__ e = l.e;
+    const T *__restrict__ r = l.r;
+    const T *__restrict__ iD = l.iD;
+    const double *__restrict__ r64 = f.r;
+    const double *__restrict__ q64 = f.q;
+    if (l.g.D == 3 && l.t.zb > 1) {
+        struct Pre {
+            L6<T> l;
+            T r, x, d;
+            double r64, q64;
+        };
+        tiled7<T, T>(
+            l.g, l.t, [&](long K) { return e[K]; }, [&](T q) { return q * c0; },
+            [&](long I) { return Pre{load_L6<T>(l.g, I, l.L), r[I], x[I], iD[I], r64[I], first ? 0.0 : q64[I]}; },
+            [&](const int *, long I, const Pre &pv, T c, T xm, T xp, T ym, T yp, T zm, T zp) {
+                T r1 = pv.r - mult7<T>(pv.l, c, xm, xp, ym, yp, zm, zp);
+                T d1 = ca * c + cb * (r1 * pv.d);
+                T xn = (pv.x + c) + d1;
+                x[I] = xn;
+                double ww = s * (double)xn;
+                a += ww * pv.r64;
+                if (!first) b += ww * pv.q64;
+            });
+    } else {
+        DISPATCH_D(l.g, {
+            struct V {
+                S7<D, T> s7;
+                T r, x, d;
+                double r64, q64;
+            };
+            col_pipe(l.g, l.t,
+                     [&](const int *p, long I) {
+                         return V{ld7<D>(l.g, p, I, l.L, [&](long K) { return e[K] * c0; }), r[I], x[I], iD[I],
+                                  r64[I], first ? 0.0 : q64[I]};
+                     },
+                     [&](const int *, long I, const V &v) {
+                         T r1 = v.r - mul7<D>(v.s7);
+                         T d1 = ca * v.s7.c + cb * (r1 * v.d);
+                         T xn = (v.x + v.s7.c) + d1;
+                         x[I] = xn;
+                         double ww = s * (double)xn;
+                         a += ww * v.r64;
+                         if (!first) b += ww * v.q64;
+                     });
+        })
+    }
+    block_partial(a, part, 2, 0);
+    block_partial(b, part, 2, 1);
+}
 // partials of w·r and w·q (w = s·x_T)
 template <class T>
 __global__ void __launch_bounds__(NT) k_fcg_dots(FcgV<T> f, double *part, int first) {

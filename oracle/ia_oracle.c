@@ -596,6 +596,8 @@ struct oc_ctx {
     int tail_level;  /* first level of at most 4096 cells (the HIP library's one-workgroup tail) */
     T *cg1_u, *cg1_w, *cg1_s; /* mg_pcg1 scratch, level-1 sized */
     int poisson_mode, smooth_it;
+    int fine_cheb;   /* the flexible CG's fine smoother: mg_cheb2 (else smooth_it pcg! iterations) */
+    T *cheb_d0;      /* its d₀ (level-0 sized) */
     double poisson_rtol;
     int last_iters[2];
     char msg[512];
@@ -868,6 +870,7 @@ static int mg_build(oc_ctx *c) {
     }
     size_t n1 = c->lev[1].g.ns;
     c->cg1_u = alloc_fill(n1, 0);
+    c->cheb_d0 = alloc_fill(c->lev[0].g.ns, 0);
     c->cg1_w = alloc_fill(n1, 0);
     c->cg1_s = alloc_fill(n1, 0);
     return OC_OK;
@@ -1580,6 +1583,7 @@ oc_ctx *oc_create(const oc_params *p) {
         c->fcg_cit[l] = (l == 1 && n > 1000000) ? 6 : n > 100000 ? 48 : n > 512 ? 64 : 32;
     }
     c->fcg_cg1 = 1;
+    c->fine_cheb = 1;
     c->poisson_mode = p->poisson_mode;
     c->poisson_rtol = p->poisson_rtol > 0 ? p->poisson_rtol : 1e-6;
     c->smooth_it = p->smooth_it;
@@ -1614,7 +1618,7 @@ void oc_destroy(oc_ctx *c) {
     T *bufs[] = {c->u, c->u0, c->fv, c->p, c->sig, c->mu0, c->f, c->f0, c->al, c->nh, c->ff, c->ru, c->ruf, c->dr};
     for (unsigned q = 0; q < sizeof bufs / sizeof bufs[0]; ++q) free(bufs[q]);
     free(c->cb);
-    free(c->cg1_u); free(c->cg1_w); free(c->cg1_s);
+    free(c->cg1_u); free(c->cg1_w); free(c->cg1_s); free(c->cheb_d0);
     for (int l = 0; l < c->nlev; ++l) {
         lev_t *lv = &c->lev[l];
         free(lv->Dg); free(lv->iD); free(lv->eps); free(lv->r);
@@ -1875,6 +1879,7 @@ int g_cycle_type = 0, g_npre = 2, g_npost = 2, g_ncoarse = 16;
 double g_omega = 0.8;
 void oc_set_coarse_it(oc_ctx *c, int it) { c->coarse_it = it; }
 void oc_set_fcg_cg1(oc_ctx *c, int on) { c->fcg_cg1 = on; }
+void oc_set_fine_cheb(oc_ctx *c, int on) { c->fine_cheb = on; }
 void oc_set_fcg_cit(oc_ctx *c, const int *cit, int n) {
     for (int l = 0; l < 16; ++l) c->fcg_cit[l] = l < n ? cit[l] : cit[n - 1];
 }
@@ -1885,6 +1890,26 @@ void oc_set_cycle(int type, int npre, int npost, double omega, int ncoarse) {
  * MultiLevelPoisson V-cycle (+ `smooth_it` pcg! iterations on the finest
  * level) as the preconditioner.  This build's solver for the 1e-6 relative
  * residual target; no reference counterpart.  Returns the iteration count. */
+/* The fine-level smoother after the V-cycle (the HIP library's k_cheb2_dots):
+ * two Chebyshev steps on D⁻¹A for the interval [0.5, 2]: d₀ = c0·(r·iD),
+ * r₁ = r − A d₀, d₁ = ca·d₀ + cb·(r₁·iD), x = (x + d₀) + d₁, with
+ * θ = 1.25, δ = 0.75, σ = θ/δ, ρ₀ = 1/σ, ρ₁ = 1/(2σ − ρ₀), c0 = 1/θ,
+ * ca = ρ₁ρ₀, cb = 2ρ₁/δ (Saad's three-term Chebyshev recurrence). */
+static void mg_cheb2(lev_t *lv, T *d0) {
+    const double lmin = 0.5, lmax = 2.0, th = (lmax + lmin) / 2, de = (lmax - lmin) / 2, sg = th / de;
+    const double ro0 = 1 / sg, ro1 = 1 / (2 * sg - ro0);
+    const T c0 = (T)(1 / th), ca = (T)(ro1 * ro0), cb = (T)(2 * ro1 / de);
+    grid_t *g = &lv->g;
+    rng_t R = r_in(g);
+    PFOR_R(R) { size_t I = IX(i, j, k); d0[I] = (lv->r[I] * lv->iD[I]) * c0; }
+    per_bc(g, d0);
+    PFOR_R(R) {
+        size_t I = IX(i, j, k);
+        T r1 = lv->r[I] - mg_mult(g, I, lv->L, lv->Dg, d0);
+        T d1 = ca * d0[I] + cb * (r1 * lv->iD[I]);
+        lv->x[I] = (lv->x[I] + d0[I]) + d1;
+    }
+}
 static void precond(oc_ctx *c, const double *r, double *w, double sc, int smooth_it) {
     lev_t *l0 = &c->lev[0];
     size_t n = c->g.ns;
@@ -1893,7 +1918,8 @@ static void precond(oc_ctx *c, const double *r, double *w, double sc, int smooth
         mg_vcycle_sym(c, 0, g_npre, g_npost, (T)g_omega, g_ncoarse);
     } else {
         mg_vcycle_it(c, 0, c->fcg_cit);
-        if (smooth_it > 0) mg_pcg(l0, smooth_it);
+        if (c->fine_cheb) mg_cheb2(l0, c->cheb_d0);
+        else if (smooth_it > 0) mg_pcg(l0, smooth_it);
     }
     per_bc(&c->g, l0->x);
     for (size_t q = 0; q < n; ++q) w[q] = sc * (double)l0->x[q];
