@@ -73,6 +73,7 @@ def lib(T):
         "oc_mg_solve": (C.c_int, [vp, C.c_double, C.c_int, ip, dp]),
         "oc_project_refined": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, ip, dp]),
         "oc_project_fcg": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, ip, dp]),
+        "oc_set_fine_cheb": (None, [vp, C.c_int]),
         "oc_getIntercept2": (R, [R, R, R]),
         "oc_getIntercept3": (R, [R, R, R, R]),
         "oc_getVolumeFraction2": (R, [R, R, R]),
@@ -290,10 +291,16 @@ class OracleSim:
         return it.value, rel.value
 
     def project_fcg(self, dt_w, rtol=1e-6, max_it=200, smooth_it=2):
+        """Flexible CG to rtol.  The fine smoother after each V-cycle is the two
+        Chebyshev steps of mg_cheb2 unless set_fine_cheb(False) selects
+        `smooth_it` pcg! iterations instead."""
         it, rel = C.c_int(), C.c_double()
         self.lib.oc_project_fcg(self.ctx, float(dt_w), float(rtol), int(max_it), int(smooth_it), C.byref(it),
                                 C.byref(rel))
         return it.value, rel.value
+
+    def set_fine_cheb(self, on=True):
+        self.lib.oc_set_fine_cheb(self.ctx, 1 if on else 0)
 
     def metrics(self, U=None, StatWL=None):
         u = np.zeros(3)

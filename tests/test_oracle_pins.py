@@ -207,9 +207,11 @@ def test_poisson_refined_reaches_tolerance():
     assert np.linalg.norm(dv - dv.mean()) < 1e-4 * np.linalg.norm(u)
 
 
-def test_poisson_fcg_matches_direct_solve():
+@pytest.mark.parametrize("cheb", [True, False])
+def test_poisson_fcg_matches_direct_solve(cheb):
     """FP64 flexible CG with the FP32 V-cycle preconditioner converges to the
-    sparse direct solution of the same operator."""
+    sparse direct solution of the same operator, with either fine smoother
+    (two Chebyshev steps, or one pcg! iteration)."""
     rng = np.random.default_rng(9)
     N = 32
     sdf = lambda x: np.sqrt((x[0] - 15.3) ** 2 + (x[1] - 12.1) ** 2) - 6.2
@@ -220,6 +222,7 @@ def test_poisson_fcg_matches_direct_solve():
     sim.lib.oc_update_L(sim.ctx, 0)
     sim.lib.oc_mg_update(sim.ctx)
     z = _div(sim.get(O.U).astype(np.float64), N)
+    sim.set_fine_cheb(cheb)
     it, rel = sim.project_fcg(1.0, 1e-8, 200, 1)
     assert rel <= 1e-8
     A = _poisson_matrix(sim)
