@@ -1484,11 +1484,11 @@ constexpr int PCG_SLOTS = 3 * CIT_MAX + 8;
 // interior cells.  Level 1 above 1M cells is bandwidth-bound and costs like
 // the fine level, so it keeps WaterLily's 6; deeper levels cost little more
 // than their launches per iteration, and solving their problems more exactly
-// cuts the outer iterations (256^3 pool: 39 -> 8.8 per solve; 512^3 slab:
-// 32 -> 15 with 48 instead of 6 on its second level, 128^3).  Levels of at
+// cuts the outer iterations (256^3 pool: 39 -> 8.5 per solve; 512^3 slab:
+// 32 -> 12 with 56 instead of 6 on its second level, 128^3).  Levels of at
 // most 512 cells (8^3 and below) keep 32: more there bought nothing.
 __host__ __device__ inline int fcg_coarse_its(long n, int l) {
-    return (l == 1 && n > 1000000) ? 6 : n > 100000 ? 48 : n > 512 ? 64 : 32;
+    return (l == 1 && n > 1000000) ? 6 : n > 100000 ? 56 : n > 512 ? 64 : 32;
 }
 constexpr size_t TAIL_LDS_MAX = 152 * 1024;  // of the 160 KiB of LDS per CU
 template <class T> struct TailLev {

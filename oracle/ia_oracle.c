@@ -1580,7 +1580,7 @@ oc_ctx *oc_create(const oc_params *p) {
     for (int l = 0; l < 16; ++l) {
         long n = 1;
         for (int a = 0; a < p->D; ++a) n *= (p->N[a] >> l) > 1 ? (p->N[a] >> l) : 1;
-        c->fcg_cit[l] = (l == 1 && n > 1000000) ? 6 : n > 100000 ? 48 : n > 512 ? 64 : 32;
+        c->fcg_cit[l] = (l == 1 && n > 1000000) ? 6 : n > 100000 ? 56 : n > 512 ? 64 : 32;
     }
     c->fcg_cg1 = 1;
     c->fine_cheb = 1;
