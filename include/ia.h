@@ -50,9 +50,10 @@ enum ia_limiter {
  * PLAIN:     plain multigrid to a relative residual: FP32 MultiLevelPoisson
  *            V-cycles (+ the fine pcg!, i.e. solver!'s loop body) inside an FP64
  *            iterative-refinement loop; stop when ||z - A x||2 / ||z||2 <= rtol.
- * REFINED:   FP64 flexible CG (FCG(1)) preconditioned by one FP32 V-cycle (+ one
- *            fine pcg! iteration) per iteration; same stopping test, FP64
- *            residual and pressure accumulation.  No reference counterpart.
+ * REFINED:   FP64 flexible CG (FCG(1)) preconditioned by one FP32 V-cycle (+ two
+ *            Chebyshev smoothing steps on the finest level) per iteration;
+ *            same stopping test, FP64 residual and pressure accumulation.  No
+ *            reference counterpart.
  * LEARNED:   REFINED, whose warm start is first corrected by the learned network
  *            (ia_learned_*; no reference counterpart).  The correction is kept
  *            only when it lowers the FP64 residual.  3D, FP32, N[0] % 16 == 0,
