@@ -612,7 +612,7 @@ template <class T> struct Impl {
         LAUNCHCHK(c);
         return host_total(c, PP_FRR, l0.ct.nblk, 1, rr);
     }
-    // one flexible-CG iteration: preconditioner (V-cycle + pcg! smoothing) and
+    // one flexible-CG iteration: preconditioner (V-cycle + fine smoothing) and
     // the FP64 Krylov update.  The input residual's partial sums are in
     // buffer `rbuf` (the true residual after a restart, else the last update).
     static void fcg_body(ia_ctx *c, int first, int par, int rbuf) {
