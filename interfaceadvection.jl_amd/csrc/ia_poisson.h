@@ -10,6 +10,7 @@
 // finishing launches are needed.
 #pragma once
 #include "ia_kernels.h"
+#include <type_traits>
 
 namespace ia {
 
@@ -276,7 +277,13 @@ __device__ inline void tiled7(const Grid &g, const CTiling &t, LF load, CF comb,
     int hxc = tx == 0 ? 0 : TX + 1, hyr = ty == 0 ? 0 : TY + 1;
     hxq = per_img(g, 0, hxq);
     hyq = per_img(g, 1, hyq);
-    auto LD = [&](int q0, int q1, int q2) { return load((long)q0 + g.st[1] * q1 + g.st[2] * q2); };
+    // load(K) takes the linear index; a load(q0, q1, q2) takes the cell's coordinates
+    auto LD = [&](int q0, int q1, int q2) {
+        if constexpr (std::is_invocable_v<LF, int, int, int>)
+            return load(q0, q1, q2);
+        else
+            return load((long)q0 + g.st[1] * q1 + g.st[2] * q2);
+    };
     R rm{}, rc{}, rp{}, rhx{}, rhy{};
     if (inarr) {
         rm = LD(xi, yi, per_img(g, 2, zlo - 1));
@@ -454,6 +461,36 @@ __device__ inline void prolong_inc_init_d(const LevV<T> &l, const Grid &gc, cons
     T *__restrict__ e = l.e;
     const T *__restrict__ iD = l.iD;
     const Grid &g = l.g;
+    if (D == 3 && l.t.zb > 1) {
+        // column tilings: the prolonged coarse x through an LDS plane tile
+        struct Pre {
+            L6<T> l;
+            T r, x, d;
+        };
+        tiled7<T, T>(
+            g, l.t,
+            [&](int q0, int q1, int q2) {
+                return xc[(long)((q0 + 3) / 2 - 1) + gc.st[1] * ((q1 + 3) / 2 - 1) + gc.st[2] * ((q2 + 3) / 2 - 1)];
+            },
+            [&](T q) { return q; },
+            [&](long I) {
+                Pre w;
+                w.l = load_L6<T>(g, I, l.L);
+                w.r = r[I];
+                w.x = x[I];
+                w.d = iD[I];
+                return w;
+            },
+            [&](const int *, long I, const Pre &pv, T c, T xm, T xp, T ym, T yp, T zm, T zp) {
+                T rr = pv.r - mult7<T>(pv.l, c, xm, xp, ym, yp, zm, zp);
+                r[I] = rr;
+                x[I] = pv.x + c;
+                T zz = rr * pv.d;
+                e[I] = zz;
+                v += (double)rr * (double)zz;
+            });
+        return;
+    }
     auto ec = [&](const int q[3]) {
         long Kc = (long)((q[0] + 3) / 2 - 1);
         if (D > 1) Kc += gc.st[1] * ((q[1] + 3) / 2 - 1);
