@@ -255,6 +255,40 @@ def test_project_reference(T, per):
     g.close()
 
 
+@pytest.mark.parametrize("per", [(), (1, 2, 3)])
+def test_project_reference_3d_tiled(per):
+    """WaterLily's solver! in FP64 at 96^3: level 0 takes the LDS-tiled
+    kernels (4-cell columns; the second x tile is half filled), which the FP64
+    projection otherwise never reaches.  Velocity scale 0.05 ends both solves
+    at Σr² ≈ 9.25e-5 (35 and 31 V-cycles at about 0.93 per cycle), well clear
+    of the 1e-4 stop, so round-off cannot move the iteration count."""
+    N = 96
+    say(f"myproject! 3D {N}^3 float64", "periodic" if per else "walls")
+    case = bubbles3d(N, np.float64)
+    if per:
+        case["kw"]["perdir"] = per
+        case["kw"]["g"] = None
+    g, o = pair(case)
+    rng = np.random.default_rng(11)
+    u = np.zeros(g.Ng + (3,), dtype=np.float64, order="F")
+    u[1:-1, 1:-1, 1:-1] = rng.standard_normal((N, N, N, 3)) * 0.05
+    g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
+    o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
+    g._call("ia_update_L", 0)
+    g._call("ia_mg_update")
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    st = ia.IASolveStats()
+    g._call("ia_project", 0.5, ia.C.byref(st))
+    it, r2 = o.project(0.5)
+    print(f"gpu {st.iters} V-cycles, oracle {it} (Σr² {r2:.3e})")
+    assert st.iters == it
+    pg, po = O.inside(g.flow.p), O.inside(o.p)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= 1e-9 * max(1.0, np.abs(po).max())
+    assert maxdiff(g.flow.u, o.u) <= 1e-9
+    g.close()
+
+
 @pytest.mark.parametrize("N", [64, 256])
 def test_project_refined_reaches_1e6(N):
     """At 256^2 the 128^2 level runs the single-reduction CG (above the tail)."""
