@@ -1034,10 +1034,17 @@ template <class T> struct Impl {
         const T *in[3] = {phi, s1, s2};
         T *out[3] = {s1, s2, phi};
         const T al[3] = {T(0), (T)(3.0 / 4), (T)(1.0 / 3)};
+        // 3D grids with multi-cell columns take the LDS-tiled stage
+        CTiling ct = ctiling(g);
+        bool tiled = g.D == 3 && ct.zb > 1;
         for (long it = 0; it < itmx; ++it)
             for (int q = 0; q < 3; ++q) {
                 prof_begin(c, IA_PROF_REDIST);
-                hipLaunchKernelGGL(k, blocks(c->t), dim3(NT), 0, c->s, g, c->t, in[q], phi, ini, out[q], dt, al[q]);
+                if (tiled)
+                    hipLaunchKernelGGL(k_ls_stage_tiled<T>, dim3((unsigned)ct.nblk), dim3(NT), 0, c->s, g, ct, in[q],
+                                       phi, ini, out[q], dt, al[q]);
+                else
+                    hipLaunchKernelGGL(k, blocks(c->t), dim3(NT), 0, c->s, g, c->t, in[q], phi, ini, out[q], dt, al[q]);
                 prof_end(c, IA_PROF_REDIST);
                 LAUNCHCHK(c);
             }

@@ -325,6 +325,81 @@ __device__ inline void tiled7(const Grid &g, const CTiling &t, LF load, CF comb,
         pv = pvn;
     }
 }
+// A position up to two cells outside the grid, as a level-set stage reads it:
+// -1 and Ng are the points two cells beyond the first and last interior cells,
+// the wrapped interior point on a periodic axis and the ghost itself
+// (mirrored) otherwise; 0 and Ng-1 are the ghosts, every other q itself.
+__device__ inline int ls_img2(const Grid &g, int a, int q) {
+    if (q == -1) return g.per[a] ? g.Ng[a] - 3 : 0;
+    if (q == g.Ng[a]) return g.per[a] ? 2 : g.Ng[a] - 1;
+    return q;
+}
+// One stage of redistaning! in 3D on a column tiling: the block's 64x4
+// columns walk their z range; each plane of ϕ with a two-cell x/y halo goes
+// through LDS (double-buffered, one barrier per plane) and the four z
+// neighbours stay in registers.  Every cell evaluates the same values in the
+// same order as k_ls_stage.
+template <class T>
+__global__ void __launch_bounds__(NT) k_ls_stage_tiled(Grid g, CTiling t, const T *phi, const T *phi0, const T *ini,
+                                                       T *out, T dtau, T alpha) {
+    constexpr int WX = TX + 4, WY = TY + 4, NW = WX * WY;
+    __shared__ T tile[2][WY][WX];
+    long b = swizzle_block((long)blockIdx.x, t.nblk);
+    long bx = b % t.nt[0], rr = b / t.nt[0];
+    long by = rr % t.nt[1], bz = rr / t.nt[1];
+    int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    int x0 = 1 + (int)bx * TX, y0 = 1 + (int)by * TY;
+    int x = x0 + tx, y = y0 + ty;
+    int zlo = 1 + (int)bz * t.zb, zhi = min(zlo + t.zb - 1, g.Ng[2] - 2);
+    bool active = x <= g.Ng[0] - 2 && y <= g.Ng[1] - 2;
+    // tile entries k = threadIdx.x + NT*j: positions (x0-2+i, y0-2+j') mapped by
+    // ls_img2; entries beyond Ng in x or y are read by no interior cell
+    long off[3];
+    bool ok[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        int k = (int)threadIdx.x + NT * j;
+        int xx = x0 - 2 + k % WX, yy = y0 - 2 + k / WX;
+        ok[j] = k < NW && xx <= g.Ng[0] && yy <= g.Ng[1];
+        off[j] = ok[j] ? (long)ls_img2(g, 0, xx) + g.st[1] * ls_img2(g, 1, yy) : 0;
+    }
+    long col = (long)x + g.st[1] * y;
+    auto LZ = [&](int z) { return active ? phi[col + g.st[2] * ls_img2(g, 2, z)] : T(0); };
+    T zm2 = LZ(zlo - 2), zm1 = LZ(zlo - 1), zc = LZ(zlo), zp1 = LZ(zlo + 1), zp2 = LZ(zlo + 2);
+    for (int z = zlo; z <= zhi; ++z) {
+        int buf = (z - zlo) & 1;
+        long zo = g.st[2] * z;
+        T *tb = &tile[buf][0][0];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int k = (int)threadIdx.x + NT * j;
+            if (k < NW) tb[k] = ok[j] ? phi[off[j] + zo] : T(0);
+        }
+        T zn = z < zhi ? LZ(z + 3) : T(0);
+        __syncthreads();
+        if (active) {
+            long I = col + zo;
+            T si = ini[I];
+            T s = si > 0 ? T(1) : (si < 0 ? T(-1) : si);
+            const T(*w)[WX] = tile[buf];
+            int cx = tx + 2, cy = ty + 2;
+            T c = w[cy][cx];
+            T acc = 0;
+            acc = acc + ls_grad2(w[cy][cx - 2], w[cy][cx - 1], c, w[cy][cx + 1], w[cy][cx + 2], s);
+            acc = acc + ls_grad2(w[cy - 2][cx], w[cy - 1][cx], c, w[cy + 1][cx], w[cy + 2][cx], s);
+            acc = acc + ls_grad2(zm2, zm1, c, zp1, zp2, s);
+            T L = si * (T(1) - sqrt(acc));
+            T v = alpha * phi0[I] + (T(1) - alpha) * (c + dtau * L);
+            int p[3] = {x, y, z};
+            write_images<T>(g, out, p, v, 0, 0, T(0));
+        }
+        zm2 = zm1;
+        zm1 = zc;
+        zc = zp1;
+        zp1 = zp2;
+        zp2 = zn;
+    }
+}
 // the L values of a 3D cell (faces I and I+δ per axis)
 template <class V> struct L6 {
     V m0, m1, m2, p0, p1, p2;

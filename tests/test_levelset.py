@@ -238,16 +238,19 @@ def test_planar_front_on_the_device(T):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("D,T,perdir,d,dτ", [
-    (2, np.float32, (), 5, 0.5),
-    (2, np.float64, (1,), 3, 0.25),
-    (3, np.float32, (1, 3), 4, 0.5),
-    (3, np.float64, (), 2, 0.5),
+@pytest.mark.parametrize("D,T,perdir,d,dτ,N", [
+    (2, np.float32, (), 5, 0.5, 48),
+    (2, np.float64, (1,), 3, 0.25, 48),
+    (3, np.float32, (1, 3), 4, 0.5, 32),
+    (3, np.float64, (), 2, 0.5, 32),
+    # 96^3 takes the LDS-tiled stage (4-cell columns, a half-filled second x tile)
+    (3, np.float32, (1, 2, 3), 2, 0.5, 96),
+    (3, np.float64, (2,), 2, 0.5, 96),
 ])
-def test_redistance_matches_the_oracle(D, T, perdir, d, dτ):
+def test_redistance_matches_the_oracle(D, T, perdir, d, dτ, N):
     from test_gpu_parity import bubble2d, bubbles3d, ia, say
-    say(f"redistaning! {D}D {np.dtype(T).name} perdir={perdir} d={d} dτ={dτ}")
-    case = bubble2d(48, T) if D == 2 else bubbles3d(32, T)
+    say(f"redistaning! {D}D {N} {np.dtype(T).name} perdir={perdir} d={d} dτ={dτ}")
+    case = bubble2d(N, T) if D == 2 else bubbles3d(N, T)
     sim = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])
     f = sim.intf.f
     ls = ia.LevelSet(sim)
