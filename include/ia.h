@@ -121,6 +121,10 @@ typedef struct ia_step_stats {
     int status;            /* worst status over the step's sweeps */
     int fill_warnings;     /* number of sweeps that reported a fill error */
     ia_solve_stats solve[2]; /* predictor, corrector projections */
+    double vol;            /* Σf over the interior after the call, accumulated in FP64 on the device */
+    double fmin, fmax;     /* min and max of f over the interior after the call (of the final
+                              field; reportFillError's findmax/findmin, advection.jl:146-147,
+                              see each sweep's field before cleanWisp!) */
 } ia_step_stats;
 
 /* TwoPhaseSimulation / cVOF construction (InterfaceAdvection.jl:67-85,
@@ -154,9 +158,16 @@ int ia_cfl(ia_ctx *c, double *dt);
  * selects the sweep order (flow.jl:163).  stats->dt_next is the Δt to push. */
 int ia_mom_step(ia_ctx *c, double dt, int step_index, ia_step_stats *stats);
 
-/* advect!(flow, intf, f, u⁰, u, dt)  (advection.jl:17-23, :34-78): pure
- * cVOF advection of f with the average of u⁰ and u. */
-int ia_advect(ia_ctx *c, double dt, int step_index, ia_step_stats *stats);
+/* advect!(a::Flow, c::cVOF, f=c.f, u¹=a.u⁰, u²=a.u, dt=last(a.Δt))
+ * (advection.jl:17-23, :34-78): pure cVOF advection of the field `f` with
+ * the average of `u1` and `u2`.  f is IA_FIELD_F or IA_FIELD_F0 (advected in
+ * place; the other one is untouched); u1 and u2 are any of IA_FIELD_U,
+ * IA_FIELD_U0, IA_FIELD_FORCE, IA_FIELD_RHOU (a caller that advects with its
+ * own velocity uploads it into one of them, or writes it through
+ * ia_field_device_ptr).  The reference's defaults are
+ * ia_advect(c, IA_FIELD_F, IA_FIELD_U0, IA_FIELD_U, dt, length(Δt), &st).
+ * stats->vol/fmin/fmax describe the advected field; dt_next = dt. */
+int ia_advect(ia_ctx *c, int f, int u1, int u2, double dt, int step_index, ia_step_stats *stats);
 
 /* Stage entry points (each one reference routine, for callers that drive the
  * step themselves and for stage-level parity tests). */

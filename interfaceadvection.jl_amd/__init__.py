@@ -44,7 +44,7 @@ class IASolveStats(C.Structure):
 
 class IAStepStats(C.Structure):
     _fields_ = [("dt_next", C.c_double), ("status", C.c_int), ("fill_warnings", C.c_int),
-                ("solve", IASolveStats * 2)]
+                ("solve", IASolveStats * 2), ("vol", C.c_double), ("fmin", C.c_double), ("fmax", C.c_double)]
 
 
 class IALearnedInfo(C.Structure):
@@ -78,7 +78,7 @@ def lib():
         "ia_set_initial_velocity": (C.c_int, [vp, vp, sz]),
         "ia_cfl": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "ia_mom_step": (C.c_int, [vp, C.c_double, C.c_int, C.POINTER(IAStepStats)]),
-        "ia_advect": (C.c_int, [vp, C.c_double, C.c_int, C.POINTER(IAStepStats)]),
+        "ia_advect": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(IAStepStats)]),
         "ia_reconstruct": (C.c_int, [vp]),
         "ia_vof_sweep": (C.c_int, [vp, C.c_int, C.c_double, C.POINTER(IAStepStats)]),
         "ia_visc": (C.c_int, [vp, C.c_int]),
@@ -497,18 +497,33 @@ def sim_info(sim):
     print(f"    tU/L={sim.sim_time():10.6f}, ΔtU/L={sim.flow.Δt[-1] * sim.U / sim.L:.10f}")
 
 
-def advect_(flow, intf=None, f=None, u1=None, u2=None, dt=None):
-    """advect!(flow, intf, f=intf.f, u¹=flow.u⁰, u²=flow.u, dt=last(flow.Δt))  (advection.jl:17-23)."""
+_ADVECT_F = {"f": FIELD["f"], "f0": FIELD["f0"]}
+_ADVECT_U = {"u": FIELD["u"], "u0": FIELD["u0"], "force": FIELD["force"], "rhou": FIELD["rhou"]}
+
+
+def advect_(flow, intf=None, f="f", u1="u0", u2="u", dt=None):
+    """advect!(a, c, f=c.f, u¹=a.u⁰, u²=a.u, dt=last(a.Δt))  (advection.jl:17-23).
+
+    f names the advected field ("f" = intf.f or "f0" = intf.f⁰, advected in
+    place); u1 and u2 name the velocities ("u", "u0", "force", "rhou").  An
+    array instead of a name is first uploaded into the default field of that
+    argument (intf.f, flow.u⁰, flow.u)."""
     sim = flow._sim if isinstance(flow, _Fields) else flow
-    if f is not None:
-        sim.intf.f = f
-    if u1 is not None:
-        sim.flow.u0 = u1
-    if u2 is not None:
-        sim.flow.u = u2
+
+    def pick(v, table, default, setter):
+        if isinstance(v, str):
+            if v not in table:
+                raise ValueError(f"advect!: {v!r} is not one of {sorted(table)}")
+            return table[v]
+        setter(v)
+        return table[default]
+
+    fid = pick(f, _ADVECT_F, "f", lambda a: setattr(sim.intf, "f", a))
+    u1id = pick(u1, _ADVECT_U, "u0", lambda a: setattr(sim.flow, "u0", a))
+    u2id = pick(u2, _ADVECT_U, "u", lambda a: setattr(sim.flow, "u", a))
     dt = sim.flow.Δt[-1] if dt is None else dt
     st = IAStepStats()
-    sim._call("ia_advect", float(dt), len(sim.flow.Δt), C.byref(st))
+    sim._call("ia_advect", fid, u1id, u2id, float(dt), len(sim.flow.Δt), C.byref(st))
     sim.last_stats = st
     return st
 

@@ -346,7 +346,7 @@ template <class T> struct Impl {
                 HIPCHK(c, hipFuncSetAttribute((const void *)k_mg_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                               (int)c->tail_lds));
         }
-        c->part_n = 2 * maxblk + 64;
+        c->part_n = 5 * maxblk + 64;
         HIPCHK(c, hipMalloc(&c->part, c->part_n * sizeof(double)));
         HIPCHK(c, hipMalloc(&c->fpart, (maxblk + FILL_R2) * sizeof(FillPart)));
         HIPCHK(c, hipMalloc(&c->st, sizeof(StepState)));
@@ -983,14 +983,18 @@ template <class T> struct Impl {
     }
 
     // ----------------------------------------------------------------- CFL
-    static int cfl(ia_ctx *c, double *out) {
+    // vol (optional): Σf in FP64, min f and max f over the interior of the current f
+    static int cfl(ia_ctx *c, double *out, double *vol = nullptr) {
         Grid &g = c->g;
-        hipLaunchKernelGGL(k_cfl<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(c->u), c->part);
-        hipLaunchKernelGGL(k_fin_max2, dim3(1), dim3(NT), 0, c->s, c->part, c->t.nblk, c->dsc + 8);
+        hipLaunchKernelGGL(k_cfl<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(c->u), P<T>(c->fb[c->fi]),
+                           c->part);
+        hipLaunchKernelGGL(k_fin_cfl, dim3(1), dim3(NT), 0, c->s, c->part, c->t.nblk, c->dsc + 8);
         LAUNCHCHK(c);
-        double mx[2];
-        HIPCHK(c, hipMemcpyAsync(mx, c->dsc + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        double mx[5];
+        HIPCHK(c, hipMemcpyAsync(mx, c->dsc + 8, 5 * sizeof(double), hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));
+        if (vol)
+            for (int q = 0; q < 3; ++q) vol[q] = mx[2 + q];
         T nu = (T)c->P.nu, lm = (T)c->P.lambda_mu, lr = (T)c->P.lambda_rho;
         T dtAdv = T(1) / ((T)mx[0] + 5 * nu);
         T dtV = T(1) / (2 * (T)mx[1]);
@@ -1273,33 +1277,54 @@ template <class T> struct Impl {
         c->u0 = U1;
         c->fi = fnew;
         c->f0i = fn;
-        double cf;
-        if (int e = cfl(c, &cf)) return e;
+        double cf, vol[3];
+        if (int e = cfl(c, &cf, vol)) return e;
         int st = read_state(c, ss);
         double nx = std::min(cf, 1.2 * (double)dt);
-        if (ss) ss->dt_next = (double)(T)nx;
+        if (ss) {
+            ss->dt_next = (double)(T)nx;
+            ss->vol = vol[0];
+            ss->fmin = vol[1];
+            ss->fmax = vol[2];
+        }
         return st;
     }
-    // advect! (advection.jl:17-78)
-    static int advect(ia_ctx *c, double dtd, int step_index, ia_step_stats *ss) {
+    // advect!(a, c, f, u¹, u², dt) (advection.jl:17-78): f is cVOF.f or cVOF.f⁰
+    // (pool index fsrc; the other one is kept), u¹ and u² any of the
+    // context's vector fields that the sweeps do not write
+    static int advect(ia_ctx *c, int fsrc, const void *u1, const void *u2, double dtd, int step_index,
+                      ia_step_stats *ss) {
         Grid &g = c->g;
         T dt = (T)dtd;
+        int keep = fsrc == c->fi ? c->f0i : c->fi;
         HIPCHK(c, hipMemsetAsync(c->st, 0, sizeof(StepState), c->s));
         HIPCHK(c, hipMemsetAsync(c->ruf, 0, g.ns * g.D * sizeof(T), c->s));
-        prepare(c, c->fb[c->fi], c->u0, false);
+        prepare(c, c->fb[fsrc], u1, false);
         int dirO[3];
         dir_order(g.D, step_index, dirO);
-        int cur = c->fi;
+        int cur = fsrc;
         for (int op = 0; op < g.D; ++op) {
             int d = dirO[op];
-            int out = free_fbuf(c, cur, c->f0i, -1);
-            vof_sweep(c, c->fb[cur], c->fb[out], c->u0, c->u, d, dt, 1, 10 * epsT(), nullptr, nullptr);
+            int out = free_fbuf(c, cur, keep, -1);
+            vof_sweep(c, c->fb[cur], c->fb[out], u1, u2, d, dt, 1, 10 * epsT(), nullptr, nullptr);
             cur = out;
         }
         LAUNCHCHK(c);
-        c->fi = cur;
+        if (fsrc == c->fi) c->fi = cur;
+        else c->f0i = cur;
+        double cf, vol[3];
+        int fi_save = c->fi;
+        c->fi = cur;  // the statistics describe the advected field
+        int e = cfl(c, &cf, vol);
+        c->fi = fi_save;
+        if (e) return e;
         int st = read_state(c, ss);
-        if (ss) ss->dt_next = dtd;
+        if (ss) {
+            ss->dt_next = dtd;
+            ss->vol = vol[0];
+            ss->fmin = vol[1];
+            ss->fmax = vol[2];
+        }
         return st;
     }
     static int vof_sweep_stage(ia_ctx *c, int d, double dtd, ia_step_stats *ss) {
@@ -1553,10 +1578,21 @@ int ia_mom_step(ia_ctx *c, double dt, int step_index, ia_step_stats *stats) {
     if (stats) memset(stats, 0, sizeof *stats);
     return dispatch(c, [&](auto impl) { return decltype(impl)::mom_step(c, dt, step_index, stats); });
 }
-int ia_advect(ia_ctx *c, double dt, int step_index, ia_step_stats *stats) {
-    if (c) c->ls_valid = false;
+int ia_advect(ia_ctx *c, int f, int u1, int u2, double dt, int step_index, ia_step_stats *stats) {
+    if (!c) return IA_ERR_BADARG;
     if (stats) memset(stats, 0, sizeof *stats);
-    return dispatch(c, [&](auto impl) { return decltype(impl)::advect(c, dt, step_index, stats); });
+    if (f != IA_FIELD_F && f != IA_FIELD_F0)
+        return fail(c, IA_ERR_BADARG, "ia_advect: f must be IA_FIELD_F or IA_FIELD_F0 (got %d)", f);
+    auto vel_ok = [](int id) {
+        return id == IA_FIELD_U || id == IA_FIELD_U0 || id == IA_FIELD_FORCE || id == IA_FIELD_RHOU;
+    };
+    if (!vel_ok(u1) || !vel_ok(u2))
+        return fail(c, IA_ERR_BADARG, "ia_advect: u1, u2 must be IA_FIELD_U, _U0, _FORCE or _RHOU (got %d, %d)", u1,
+                    u2);
+    c->ls_valid = false;
+    int fsrc = f == IA_FIELD_F ? c->fi : c->f0i;
+    const void *b1 = field_buf(c, u1), *b2 = field_buf(c, u2);
+    return dispatch(c, [&](auto impl) { return decltype(impl)::advect(c, fsrc, b1, b2, dt, step_index, stats); });
 }
 int ia_reconstruct(ia_ctx *c) {
     return dispatch(c, [&](auto impl) { return decltype(impl)::reconstruct_stage(c); });

@@ -639,6 +639,38 @@ template <int D, class T> __device__ inline double vort_edge(const Grid &g, cons
     int j = (a + 1) % 3, k = (a + 2) % 3;
     return (double)((u[g.ns * k + I] - u[g.ns * k + I - g.st[j]]) - (u[g.ns * j + I] - u[g.ns * j + I - g.st[k]]));
 }
+// the metrics.jl integrands of interior cell p added into acc[0..6]
+template <int D, class T>
+__device__ inline void metrics_cell(const Grid &g, const T *u, const T *f, T lr, const double U[3], const double gv[3],
+                                    const double wl[3], const int p[3], double acc[7]) {
+    long I = lin(g, p);
+    double rho = (double)linp(f[I], lr);
+    double ke = 0, pe = 0;
+    for (int i = 0; i < D; ++i) {
+        const T *ui = u + g.ns * i;
+        double a = (double)ui[I] - U[i], b = (double)ui[I + g.st[i]] - U[i];
+        ke += (a * a + b * b) * rho;
+        acc[1 + i] += 0.5 * (((double)ui[I] + (double)ui[I + g.st[i]]) - 2 * U[i]) * rho;
+        pe += gv[i] * ((p[i] - 0.5) - wl[i]);
+    }
+    acc[0] += 0.25 * ke;
+    acc[4] += -rho * pe;
+    double en = 0;
+    if (D == 2) {
+        double a = vort_edge<D>(g, u, I, 0), b = vort_edge<D>(g, u, I + g.st[0], 0),
+               c = vort_edge<D>(g, u, I + g.st[1], 0), d = vort_edge<D>(g, u, I + g.st[0] + g.st[1], 0);
+        en = ((a * a + b * b) + c * c) + d * d;
+    } else {
+        for (int i = 0; i < 3; ++i) {
+            int ix = (i + 1) % 3, iy = (i + 2) % 3;
+            double a = vort_edge<D>(g, u, I, i), b = vort_edge<D>(g, u, I + g.st[ix], i),
+                   c = vort_edge<D>(g, u, I + g.st[iy], i), d = vort_edge<D>(g, u, I + g.st[ix] + g.st[iy], i);
+            en += ((a * a + b * b) + c * c) + d * d;
+        }
+    }
+    acc[5] += 0.5 * 0.25 * en;
+    acc[6] += (double)f[I];
+}
 template <int D, class T>
 __global__ void __launch_bounds__(NT) k_metrics(Grid g, Tiling t, const T *u, const T *f, T lr, double U0, double U1,
                                                 double U2, double g0, double g1, double g2, double w0, double w1,
@@ -646,34 +678,8 @@ __global__ void __launch_bounds__(NT) k_metrics(Grid g, Tiling t, const T *u, co
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     int p[3];
     if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
         double U[3] = {U0, U1, U2}, gv[3] = {g0, g1, g2}, wl[3] = {w0, w1, w2};
-        double rho = (double)linp(f[I], lr);
-        double ke = 0, pe = 0;
-        for (int i = 0; i < D; ++i) {
-            const T *ui = u + g.ns * i;
-            double a = (double)ui[I] - U[i], b = (double)ui[I + g.st[i]] - U[i];
-            ke += (a * a + b * b) * rho;
-            acc[1 + i] = 0.5 * (((double)ui[I] + (double)ui[I + g.st[i]]) - 2 * U[i]) * rho;
-            pe += gv[i] * ((p[i] - 0.5) - wl[i]);
-        }
-        acc[0] = 0.25 * ke;
-        acc[4] = -rho * pe;
-        double en = 0;
-        if (D == 2) {
-            double a = vort_edge<D>(g, u, I, 0), b = vort_edge<D>(g, u, I + g.st[0], 0),
-                   c = vort_edge<D>(g, u, I + g.st[1], 0), d = vort_edge<D>(g, u, I + g.st[0] + g.st[1], 0);
-            en = ((a * a + b * b) + c * c) + d * d;
-        } else {
-            for (int i = 0; i < 3; ++i) {
-                int ix = (i + 1) % 3, iy = (i + 2) % 3;
-                double a = vort_edge<D>(g, u, I, i), b = vort_edge<D>(g, u, I + g.st[ix], i),
-                       c = vort_edge<D>(g, u, I + g.st[iy], i), d = vort_edge<D>(g, u, I + g.st[ix] + g.st[iy], i);
-                en += ((a * a + b * b) + c * c) + d * d;
-            }
-        }
-        acc[5] = 0.5 * 0.25 * en;
-        acc[6] = (double)f[I];
+        metrics_cell<D, T>(g, u, f, lr, U, gv, wl, p, acc);
     }
     for (int q = 0; q < 7; ++q) {
         double s = block_sum(acc[q]);
@@ -945,11 +951,13 @@ __global__ void __launch_bounds__(NT) k_project(Grid g, Tiling t, T *u, const T 
         write_images<T>(g, u + g.ns * i, p, un, 1, i, ub[i]);
     }
 }
-// MPCFL (flow.jl:262-281): block maxima of flux_out and maxTotalFlux
+// MPCFL (flow.jl:262-281): block maxima of flux_out and maxTotalFlux; and,
+// for the step's statistics, the block's FP64 Σf and its min / max f over
+// the interior (5 partials per block)
 template <class T>
-__global__ void __launch_bounds__(NT) k_cfl(Grid g, Tiling t, const T *u, double *part) {
+__global__ void __launch_bounds__(NT) k_cfl(Grid g, Tiling t, const T *u, const T *f, double *part) {
     int p[3];
-    double a = 0, b = 0;
+    double a = 0, b = 0, fs = 0, fmn = 1e300, fmx = -1e300;
     if (thread_cell(g, t, p)) {
         long I = lin(g, p);
         double acc = 0;
@@ -961,25 +969,56 @@ __global__ void __launch_bounds__(NT) k_cfl(Grid g, Tiling t, const T *u, double
         }
         a = (double)(T)acc;
         b = (double)s2;
+        fs = (double)f[I];
+        fmn = fs;
+        fmx = fs;
     }
     double ma = block_max(a);
     __syncthreads();
     double mb = block_max(b);
+    __syncthreads();
+    double sf = block_sum(fs);
+    __syncthreads();
+    double mx = block_max(fmx);
+    __syncthreads();
+    double mn = -block_max(-fmn);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = ma;
-        part[2 * blockIdx.x + 1] = mb;
+        double *o = part + 5 * blockIdx.x;
+        o[0] = ma;
+        o[1] = mb;
+        o[2] = sf;
+        o[3] = mn;
+        o[4] = mx;
     }
 }
-__global__ void __launch_bounds__(NT) k_fin_max2(const double *part, long nblk, double *out) {
-    double a = 0, b = 0;
+// the two CFL maxima, Σf (each thread adds its blocks in order, then a fixed
+// tree), min f and max f of the k_cfl partials
+__global__ void __launch_bounds__(NT) k_fin_cfl(const double *part, long nblk, double *out) {
+    double a = 0, b = 0, s = 0, mn = 1e300, mx = -1e300;
     for (long q = threadIdx.x; q < nblk; q += NT) {
-        a = part[2 * q] > a ? part[2 * q] : a;
-        b = part[2 * q + 1] > b ? part[2 * q + 1] : b;
+        const double *o = part + 5 * q;
+        a = o[0] > a ? o[0] : a;
+        b = o[1] > b ? o[1] : b;
+        s += o[2];
+        mn = o[3] < mn ? o[3] : mn;
+        mx = o[4] > mx ? o[4] : mx;
     }
     double ma = block_max(a);
     __syncthreads();
     double mb = block_max(b);
-    if (threadIdx.x == 0) { out[0] = ma; out[1] = mb; }
+    __syncthreads();
+    double ss = block_sum(s);
+    __syncthreads();
+    double mmx = block_max(mx);
+    __syncthreads();
+    double mmn = -block_max(-mn);
+    if (threadIdx.x == 0) {
+        out[0] = ma;
+        out[1] = mb;
+        out[2] = ss;
+        out[3] = mmn;
+        out[4] = mmx;
+    }
 }
 // f⁰ = (f⁰ + f)/2 over every cell (flow.jl:74)
 template <class T>

@@ -172,7 +172,7 @@ static inline T median3(T a, T b, T c) { return tmax(tmin(a, b), tmin(tmax(a, b)
 
 /* ------------------------------------------------------------------ */
 /* PLIC.jl (Scardovelli & Zaleski 2000)                                 */
-/* pinned by maintests.jl:43-67 KATs + round trips + closed forms        */
+/* pinned by maintests.jl:25-51 KATs + round trips + closed forms        */
 /* ------------------------------------------------------------------ */
 /* PLIC.jl:178 */
 static inline void sort2(T *a, T *b) {
@@ -450,7 +450,7 @@ static void normal_scheme(const grid_t *g, int scheme, const T *f, T *nh, size_t
 
 /* ------------------------------------------------------------------ */
 /* boundary conditions (VOFutil.jl:44-117, WaterLily BC!, perBC!)      */
-/* pinned by maintests.jl:97-147                                       */
+/* pinned by maintests.jl:79-129                                       */
 /* ------------------------------------------------------------------ */
 /* VOFutil.jl:64-74 BCf!(f) (d == 0) and :76-89 BCf!(d,f) / :106-117 BCv1D! */
 static void bc_f(const grid_t *g, T *f, int dd) {

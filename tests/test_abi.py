@@ -4,6 +4,7 @@ before anything reaches the device.  (No compute without a GPU.)"""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -36,11 +37,38 @@ def test_library_is_gfx950_only():
     assert b"gfx950" in data
 
 
-def test_params_struct_matches_header():
+def _c_layout(tmp_path, structs):
+    """sizeof and offsetof of every field, as a C compiler lays out include/ia.h."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ia.h"', "int main(void) {"]
+    for cname, fields in structs.items():
+        lines.append(f'    printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'    printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    res = {}
+    for ln in out.splitlines():
+        cname, f, v = ln.split()
+        res[(cname, f)] = int(v)
+    return res
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every field offset and every struct size of the ctypes mirror equals
+    what a C compiler gives for include/ia.h."""
     ia = load_package()
-    # D, N[3], dtype, perdir (6 ints), then 6 doubles (u_bc, g) ...
-    assert C.sizeof(ia.IAParams) >= 6 * 4 + 6 * 8
-    assert ia.IAParams.u_bc.offset % 8 == 0
+    mirror = {"ia_params": ia.IAParams, "ia_solve_stats": ia.IASolveStats, "ia_step_stats": ia.IAStepStats,
+              "ia_learned_info": ia.IALearnedInfo}
+    structs = {c: [f[0] for f in py._fields_] for c, py in mirror.items()}
+    lay = _c_layout(tmp_path, structs)
+    for cname, py in mirror.items():
+        assert lay[(cname, "sizeof")] == C.sizeof(py), cname
+        for f in structs[cname]:
+            assert lay[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_host_rejects_unsupported_plugins():

@@ -89,7 +89,7 @@ def test_single_vortex_128_full_run():
             o.set(O.U, s * u)
             o.set(O.U0, s * u)
         a = time.perf_counter()
-        g._call("ia_advect", dt, n, None)
+        g._call("ia_advect", 0, 10, 9, dt, n, None)  # f, u⁰, u
         b = time.perf_counter()
         o.advect(dt, n)
         tg += b - a

@@ -181,13 +181,50 @@ def test_advect_single_vortex(T):
     o.set(O.U0, u)
     V0 = O.inside(o.f).astype(np.float64).sum()
     for n in range(1, 25):
-        g._call("ia_advect", 0.25, n, None)
-        o.advect(0.25, n)
+        st = ia.advect_(g, dt=0.25)
+        g.flow.Δt.append(g.T.type(0.25))
+        o.advect(0.25, len(g.flow.Δt) - 1)
     fg = g.intf.f
     tol = 1e-12 if T == np.float64 else 1e-5
     assert maxdiff(fg, o.f) <= tol
     assert fg.min() >= 0 and fg.max() <= 1
     assert abs(O.inside(fg).astype(np.float64).sum() - V0) <= (1e-12 if T == np.float64 else 1e-5) * V0
+    # the call's statistics: FP64 Σf and the extrema of the advected field
+    fi = O.inside(fg).astype(np.float64)
+    assert abs(st.vol - fi.sum()) <= 1e-12 * fi.size
+    assert st.fmin == fi.min() and st.fmax == fi.max()
+    g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_advect_named_fields(T):
+    """advect!(a, c, f, u¹, u², dt) with f = c.f⁰ and the velocities taken from
+    Flow.f and cVOF.ρu (advection.jl:17): the same sweeps as the oracle's
+    advect! of that f with those velocities; c.f is left untouched."""
+    sdf = lambda x: np.sqrt((x[0] - 12.3) ** 2 + (x[1] - 19.1) ** 2) - 6.2
+    case = dict(dims=(32, 32), u_BC=(0.0, 0.0), L=32.0, kw=dict(T=T, InterfaceSDF=sdf, perdir=(1, 2)))
+    g, o = pair(case)
+    u = _swirl(case, T, amp=1.0, seed=3)
+    w = _swirl(case, T, amp=0.7, seed=4)
+    f_before = g.intf.f
+    g._set("force", u)
+    g._set("rhou", w)
+    g.intf.f0 = f_before  # the field to advect
+    o.set(O.U0, u)
+    o.set(O.U, w)
+    for n in range(1, 6):
+        g.flow.Δt.append(g.T.type(0.25))
+        st = ia.advect_(g, f="f0", u1="force", u2="rhou", dt=0.25)
+        o.advect(0.25, len(g.flow.Δt))
+    tol = 1e-12 if T == np.float64 else 1e-5
+    assert maxdiff(g.intf.f0, o.f) <= tol
+    assert np.array_equal(g.intf.f, f_before)
+    fi = O.inside(g.intf.f0).astype(np.float64)
+    assert abs(st.vol - fi.sum()) <= 1e-12 * fi.size
+    assert st.fmin == fi.min() and st.fmax == fi.max()
+    with pytest.raises(ia.IAError):
+        g._call("ia_advect", 12, 9, 10, 0.25, 1, None)  # p is not an advectable field
     g.close()
 
 
@@ -373,7 +410,7 @@ def test_project_plain_multigrid_reaches_1e6():
 
 # --------------------------------------------------------------------------- whole steps
 def test_quiescent_step_is_noop():
-    """maintests.jl:211-220 on the GPU path."""
+    """maintests.jl:193-202 on the GPU path."""
     say("quiescent droplet 8^2 FP64, sim_step!")
     case = quiescent()
     g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])
@@ -386,7 +423,7 @@ def test_quiescent_step_is_noop():
 
 
 def test_tgv_droplet_volume():
-    """maintests.jl:222-234 on the GPU path."""
+    """maintests.jl:204-215 on the GPU path."""
     say("TGV droplet 16^2 FP64, 4 steps")
     case = tgv_droplet()
     g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **case["kw"])

@@ -23,7 +23,7 @@ F = lambda a, T=np.float64: np.asfortranarray(np.asarray(a, dtype=T))
 # --------------------------------------------------------------------------- the reference's testset
 @pytest.mark.parametrize("perdir", [(), (2,)])
 def test_computeL_vanishes_on_an_exact_ramp(perdir):
-    """maintests.jl:269-281: ϕ = i - 2.5 - 8 (|∇ϕ| = 1 everywhere, edges included)."""
+    """maintests.jl:267-278: ϕ = i - 2.5 - 8 (|∇ϕ| = 1 everywhere, edges included)."""
     N = (18, 18)
     phi = F([[i - 2.5 - 8 for j in range(1, N[1] + 1)] for i in range(1, N[0] + 1)])
     ini = phi.copy(order="F")
@@ -33,7 +33,7 @@ def test_computeL_vanishes_on_an_exact_ramp(perdir):
 
 
 def test_stage_sequence_on_a_diagonal_plane():
-    """maintests.jl:283-296: three stages (α = 0, 3/4, 1/3) with ϕini aliasing ϕ."""
+    """maintests.jl:280-293: three stages (α = 0, 3/4, 1/3) with ϕini aliasing ϕ."""
     phi = F([[i + j for j in range(1, 7)] for i in range(1, 7)])
     dτ, Lval = 0.01, 1 - np.sqrt(2)
     Lb = np.zeros_like(phi)
@@ -54,7 +54,7 @@ def planar_front(N=16, T=np.float64):
 
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_planar_front_becomes_a_distance(T):
-    """maintests.jl:309-318 (d = 4, dτ = 0.05, perdir = (2,))."""
+    """maintests.jl:304-317 (d = 4, dτ = 0.05, perdir = (2,))."""
     f = planar_front(T=T)
     phi = F(2 * f - 1, T)
     ini, phi0, L = phi.copy(order="F"), np.zeros_like(phi), np.zeros_like(phi)
@@ -203,7 +203,7 @@ def _ulp_diff(a, b):
 
 @pytest.mark.gpu
 def test_levelset_construction_matches_2f_minus_1():
-    """maintests.jl:299-305: LevelSet(sim) on (8,8), SDF x -> x[1] - 4, perdir (2,)."""
+    """maintests.jl:295-302: LevelSet(sim) on (8,8), SDF x -> x[1] - 4, perdir (2,)."""
     from test_gpu_parity import ia, say
     say("LevelSet(sim) on 8^2")
     sim = ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), 8.0, T=np.float64, InterfaceSDF=lambda x: x[0] - 4,
@@ -218,7 +218,7 @@ def test_levelset_construction_matches_2f_minus_1():
 @pytest.mark.gpu
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_planar_front_on_the_device(T):
-    """maintests.jl:307-318 through the C ABI, and equal to the oracle."""
+    """maintests.jl:304-317 through the C ABI, and equal to the oracle."""
     from test_gpu_parity import ia, say
     say(f"redistaning! planar front 16^2 {np.dtype(T).name}")
     sim = ia.TwoPhaseSimulation((16, 16), (0.0, 0.0), 16.0, T=T, InterfaceSDF=lambda x: x[0] - 8, perdir=(2,))

@@ -25,7 +25,7 @@ def approx(a, b, T):
 
 @pytest.mark.parametrize("T", TS)
 def test_plic_kats(T):
-    """maintests.jl:43-67."""
+    """maintests.jl:25-51."""
     L = O.lib(T)
     t = lambda x: T(x)
     gi2, gi3 = L.oc_getIntercept2, L.oc_getIntercept3
@@ -58,7 +58,7 @@ def _blk(f):
 
 
 def test_util_and_height_kats():
-    """maintests.jl:71-82 (get3CellHeight, getρ) and :5-23 (index helpers via the normals)."""
+    """maintests.jl:53-63 (get3CellHeight, getρ) and :5-23 (index helpers via the normals)."""
     f = np.zeros((3, 3))
     f[1, 1] = 0.32
     f[1, 2] = 0.64
@@ -71,7 +71,7 @@ def test_util_and_height_kats():
 
 
 def test_getmu_kats():
-    """maintests.jl:83-88."""
+    """maintests.jl:65-70."""
     L = O.lib(np.float64)
     ff = np.zeros((3, 3, 2), order="F")
     ff[2, 1, 0] = 0.1
@@ -87,7 +87,7 @@ def test_getmu_kats():
 
 
 def test_f2face_kat():
-    """maintests.jl:90-95."""
+    """maintests.jl:72-77."""
     L = O.lib(np.float64)
     fc = np.zeros((4, 4), order="F")
     fc[1:3, 1:3] = [[0.2, 0.6], [0.4, 0.8]]
@@ -98,7 +98,7 @@ def test_f2face_kat():
 
 
 def test_bc_kats():
-    """maintests.jl:97-147 (BCf!, BCf!(d), BCv!, BCv1D!, BCVOF!)."""
+    """maintests.jl:79-129 (BCf!, BCf!(d), BCv!, BCv1D!, BCVOF!)."""
     L = O.lib(np.float64)
     rng = np.random.default_rng(0)
     Ng = O._ints([6, 6])
@@ -149,7 +149,7 @@ def test_bc_kats():
 
 
 def test_applyvof_kat():
-    """maintests.jl:149-154 (applyVOF! on a 2x2 plane) and :253-258 (cVOF ctor, ghosts)."""
+    """maintests.jl:131-136 (applyVOF! on a 2x2 plane) and :235-240 (cVOF ctor, ghosts)."""
     sdf = lambda x: (-x[0] - 3 * x[1] + 4.5) / np.sqrt(10)
     fref_inner = np.array([[0, 2 / 3], [1 / 24, 23 / 24]])
     # A 2x2 grid cannot host a MultiLevelPoisson, so the same plane is put on a
@@ -162,7 +162,7 @@ def test_applyvof_kat():
 
 
 def test_normal_kats():
-    """maintests.jl:157-174 (WY and MYC)."""
+    """maintests.jl:139-156 (WY and MYC)."""
     L = O.lib(np.float64)
     Ng = O._ints([3, 3])
     I = O._ints([2, 2])
@@ -199,7 +199,7 @@ def test_normal_pcd_youngs_slic_closed_forms():
 
 
 def test_vof_flux_kats():
-    """maintests.jl:176-198 (getVOFFlux! per face)."""
+    """maintests.jl:158-180 (getVOFFlux! per face)."""
     L = O.lib(np.float64)
     Ng = O._ints([3, 3])
     f = np.zeros((3, 3), order="F")
@@ -223,7 +223,7 @@ def test_vof_flux_kats():
 
 
 def test_cvof_ctor_kats():
-    """maintests.jl:237-259: without an SDF f ≡ 1, α = n̂ = 0; the SDF plane incl. ghosts."""
+    """maintests.jl:219-242: without an SDF f ≡ 1, α = n̂ = 0; the SDF plane incl. ghosts."""
     sim = O.OracleSim((8, 8), (0.0, 0.0), T=np.float64)
     assert (sim.f == 1).all() and (sim.get(O.ALPHA) == 0).all() and (sim.get(O.NHAT) == 0).all()
     sdf = lambda x: (-x[0] - 3 * x[1] + 4.5) / np.sqrt(10)
@@ -255,7 +255,7 @@ def _tgv(N=16, T=np.float64, λμ=0.1, λρ=0.1, **kw):
 
 
 def test_quiescent_step_is_noop():
-    """maintests.jl:211-220: zero velocity, no forcing -> u stays 0, f bit-identical, p finite."""
+    """maintests.jl:193-202: zero velocity, no forcing -> u stays 0, f bit-identical, p finite."""
     sim = _quiescent()
     f0 = sim.f.copy()
     sim.sim_step()
@@ -265,7 +265,7 @@ def test_quiescent_step_is_noop():
 
 
 def test_tgv_droplet_volume():
-    """maintests.jl:222-234: TGV droplet, Σf conserved to rtol 1e-4 over 3 steps after the first."""
+    """maintests.jl:204-215: TGV droplet, Σf conserved to rtol 1e-4 over 3 steps after the first."""
     sim = _tgv()
     sim.sim_step()
     V0 = O.inside(sim.f).sum()
@@ -276,7 +276,7 @@ def test_tgv_droplet_volume():
 
 
 def test_twophase_ctor_kat():
-    """maintests.jl:262-279 (construction only): u_BC=(1,0), periodic in y."""
+    """maintests.jl:244-261 (construction only): u_BC=(1,0), periodic in y."""
     sim = O.OracleSim((8, 8), (1.0, 0.0), 8.0, T=np.float64, InterfaceSDF=lambda x: x[0] - 4, perdir=(2,))
     assert sim.L == 8.0 and sim.U == 1.0
     f = sim.f
