@@ -1048,19 +1048,24 @@ typedef struct {
     T dt, lr;
     int lim;
 } mom_args;
+/* flow.jl:43-56: the SynDRoM face value of ϕq once Ψ and the donor's old
+ * face fraction fd = fOld[IiCell] are known */
+static T phiq_value(T Psi, T fd, T dt, T lr, int lim, T uu, T cc, T dd) {
+    T vI = cc, vd = limiter(lim, uu, cc, dd);
+    T va = 2 * vI - vd;
+    T mOut = fabs(Psi) * dt;
+    T mOld = lin(fd, lr);
+    if (mOut > mOld) return Psi * vI;
+    T l2 = fabs(mOut) / mOld, l1 = 1 - l2;
+    T vb = l2 * va + l1 * vd;
+    return (Psi * (vb + vd)) / 2;
+}
 /* flow.jl:37-57 ϕq;  II = linear vector index (cell I, comp i), I = cell index */
 static T phiq(const mom_args *m, int jd, int i, size_t Icell, size_t II, T uu, T cc, T dd) {
     const grid_t *g = m->g;
     T Psi = (VC(m->ruf, jd)[Icell] + VC(m->ruf, jd)[Icell - ST(i)]) / 2;
     size_t IiC = Psi > 0 ? II - ST(jd) : II;
-    T vI = cc, vd = limiter(m->lim, uu, cc, dd);
-    T va = 2 * vI - vd;
-    T mOut = fabs(Psi) * m->dt;
-    T mOld = lin(m->drho[IiC], m->lr);
-    if (mOut > mOld) return Psi * vI;
-    T l2 = fabs(mOut) / mOld, l1 = 1 - l2;
-    T vb = l2 * va + l1 * vd;
-    return (Psi * (vb + vd)) / 2;
+    return phiq_value(Psi, m->drho[IiC], m->dt, m->lr, m->lim, uu, cc, dd);
 }
 static inline T psi_of(const mom_args *m, int jd, int i, size_t I) {
     const grid_t *g = m->g;
@@ -2011,6 +2016,9 @@ real_t oc_f2alpha3(real_t m1, real_t m2, real_t m3, real_t v) { return f2a3(m1, 
 real_t oc_alpha2f3(real_t m1, real_t m2, real_t m3, real_t a) { return a2f3(m1, m2, m3, a); }
 real_t oc_koren(real_t u, real_t c, real_t d) { return limiter(0, u, c, d); }
 real_t oc_limiter(int lim, real_t u, real_t c, real_t d) { return limiter(lim, u, c, d); }
+real_t oc_phiq_value(real_t Psi, real_t fd, real_t dt, real_t lr, int lim, real_t u, real_t c, real_t d) {
+    return phiq_value(Psi, fd, dt, lr, lim, u, c, d);
+}
 double oc_rhoke(int D, const int *Ng, const real_t *u, const real_t *f, real_t lr, const int *I, const double *U) {
     grid_t g;
     grid_init(&g, D, Ng, 0);

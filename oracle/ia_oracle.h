@@ -111,6 +111,8 @@ int oc_vof_flux_face(int D, const int *Ng, real_t *ff, const real_t *f, const re
 real_t oc_koren(real_t u, real_t c, real_t d);
 /* flow.jl:5-15: 0 Koren, 1 minmod, 2 upwind, 3 vanAlbada1, 4 superbee, 5 TVDcen, 6 TVDdown, 7 Sweby(γ=1.5) */
 real_t oc_limiter(int lim, real_t u, real_t c, real_t d);
+/* ϕq's face value (flow.jl:43-56) for a given Ψ and donor face fraction fd */
+real_t oc_phiq_value(real_t Psi, real_t fd, real_t dt, real_t lr, int lim, real_t u, real_t c, real_t d);
 /* surfaceTension.jl: height (getPopinetHeight) and curvature (getCurvature) about
  * cell I (1-based) of a standalone block, along the signed axis i (1-based) */
 real_t oc_popinet_height(int D, const int *Ng, const real_t *f, const int *I, int i);

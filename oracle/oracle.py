@@ -82,6 +82,7 @@ def lib(T):
         "oc_alpha2f3": (R, [R, R, R, R]),
         "oc_koren": (R, [R, R, R]),
         "oc_limiter": (R, [C.c_int, R, R, R]),
+        "oc_phiq_value": (R, [R, R, R, R, C.c_int, R, R, R]),
         "oc_popinet_height": (R, [C.c_int, ip, vp, ip, C.c_int]),
         "oc_rhoke": (C.c_double, [C.c_int, ip, vp, vp, R, ip, vp]),
         "oc_rhou": (C.c_double, [C.c_int, ip, C.c_int, vp, vp, R, ip, vp]),

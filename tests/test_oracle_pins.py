@@ -120,6 +120,66 @@ def test_wh_normal_column_regime(D):
     assert hits > 50
 
 
+def _wh(f, D):
+    out = np.zeros(D)
+    O.lib(np.float64).oc_normal_block(D, O._ints([3] * D), O._ptr(np.asfortranarray(f)), O.NORMAL["WH"],
+                                      O._ints([2] * D), O._ptr(out))
+    return out
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_wh_normal_row_width_regime(D):
+    """WH's underfill and overfill branches (normalEstimation.jl:140-152) are
+    exact for a plane too: on the line s·X + Y = h (block coordinates about
+    the centre cell, dark below), with the left column and the bottom row
+    unclipped by the block, hl = h + s + 1.5 and wb = (h + 1)/s + 1.5, so
+    (hl - 0.5)/(wb - 0.5) = s; with the right column and the top row
+    unclipped, (2.5 - hr)/(2.5 - wt) = s.  The branch and its guard
+    (wb > min(1/(2 slope), 4.5 slope), normalEstimation.jl:145) test the
+    first stage's column-difference slope.  Every case is checked in the 8
+    orientations of the square (in 3D extruded along z, where the minor slope
+    in z is 0)."""
+    rng = np.random.default_rng(11)
+    hits = {"under": 0, "over": 0}
+    for _ in range(4000):
+        s = rng.uniform(0.05, 0.95)
+        h = rng.uniform(-1.5, 1.5)
+        f = _plane_block(2, (s, 1.0), h)  # f[x, y]
+        if not (0 < f[1, 1] < 1):
+            continue
+        Sh = f.sum()
+        hl, hr, wb, wt = f[0].sum(), f[2].sum(), f[:, 0].sum(), f[:, 2].sum()
+        # the branches test the first stage's slope, the column difference (hl - hr)/2
+        se = abs(hl - hr) / 2
+        mn = min(1 / (2 * se), 4.5 * se)
+        if Sh < 4.5 * se:
+            ok = (h + 1.5 * s < 1.5 and h + 0.5 * s > -1.5 and (h + 1.5) / s < 1.5 and (h + 0.5) / s > -1.5
+                  and wb > mn)
+            kind = "under"
+        elif Sh > 9 - 4.5 * se:
+            ok = (h - 0.5 * s < 1.5 and h - 1.5 * s > -1.5 and (h - 0.5) / s < 1.5 and (h - 1.5) / s > -1.5
+                  and wt < 3 - mn)
+            kind = "over"
+        else:
+            continue
+        if not ok:
+            continue
+        # the 8 orientations: flips of x and y, and the transpose
+        for fx, fy, tr in itertools.product((1, -1), (1, -1), (False, True)):
+            g = f[::fx, ::fy]
+            n = np.array([s * fx, 1.0 * fy])
+            if tr:
+                g = g.T
+                n = n[::-1]
+            if D == 3:
+                g = np.repeat(g[:, :, None], 3, axis=2)
+                n = np.append(n, 0.0)
+            out = _wh(g, D)
+            assert np.allclose(out, n, atol=1e-12), (kind, s, h, fx, fy, tr, out, n)
+        hits[kind] += 1
+    assert hits["under"] > 20 and hits["over"] > 20, hits
+
+
 def _poisson_matrix(sim):
     """Assemble A of WaterLily's mult(I,L,D,x) on the interior from μ₀ (levels[1].L)."""
     L = sim.get(O.MU0)

@@ -1,0 +1,256 @@
+"""Pins of the oracle's step functions that no reference test covers, each by
+what the mathematics fixes (CPU only, FP64 oracle):
+
+- visc! (flow.jl:113-153): uniform u gives no force, whatever the phases;
+  single phase (λμ = λρ = 1, so the bounded viscosity is μ), a quadratic u
+  gives exactly μ(∇²u_i + ∂_i ∇·u) at every interior face, wall slices
+  included; on periodic axes, Fourier modes give their exact discrete
+  eigenvalue 2cos(k) - 2 (the pure-shear mode once, the compression mode
+  twice, through the transposed gradient).
+- MPCFL (flow.jl:262-281): hand-computed cases in which each of its five
+  limits is the active one.
+- ϕq (flow.jl:37-57): hand-computed face values in the donor regime
+  (|Ψ|δt > mOld: Ψ·c), in the blend regime, and the l2 → 0 limit (Ψ times
+  the limiter's face value).
+- updateU! with gravity (flow.jl:244-252) through a whole MPFMomStep!: a flat
+  interface at rest stays at rest, and the pressure is hydrostatic,
+  p_j - p_{j-1} = ρ_face g_y on every vertical face.
+- updateU! in a periodic box (flow.jl:244-252, :74-77): free fall, u = gΔt,
+  and the interface translated by the predictor's gΔt²/2.
+- reportFillError (advection.jl:145-189): a strongly divergent velocity at the
+  interface (|∇·u⁰| + |∇·u| > 10 where f leaves [0, 1]) returns the explosion
+  status, exactly 10 only the fill warning.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+T = np.float64
+
+
+def _sim(dims, **kw):
+    return O.OracleSim(dims, (0.0,) * len(dims), float(dims[0]), T=T, **kw)
+
+
+def _face_coords(Ng, i):
+    """Coordinates of the staggered component i (0-based) at every index,
+    x_d = q_d - 0.5 δ(i, d)."""
+    D = len(Ng)
+    ax = [np.arange(n, dtype=np.float64) - (0.5 if d == i else 0.0) for d, n in enumerate(Ng)]
+    return np.meshgrid(*ax, indexing="ij")
+
+
+# ----------------------------------------------------------------------------- visc!
+@pytest.mark.parametrize("dims", [(8, 8, 8), (16, 8)])
+def test_visc_uniform_velocity_gives_no_force(dims):
+    rng = np.random.default_rng(0)
+    o = _sim(dims, ν=0.3, λμ=1e-2, λρ=1e-3)
+    D = len(dims)
+    Ng = o.Ng
+    f = np.asfortranarray(rng.uniform(0, 1, Ng))
+    o.set(O.F, f)
+    u = np.zeros(Ng + (D,), order="F")
+    for i in range(D):
+        u[..., i] = 0.37 * (i + 1) - 0.5
+    o.set(O.U, u)
+    o.lib.oc_visc(o.ctx, 0)
+    r = o.get(O.FORCE)
+    assert np.abs(r).max() == 0.0
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8), (16, 8)])
+def test_visc_single_phase_quadratic_is_exact(dims):
+    """r_i = μ (tr H_i + Σ_j (H_j)_{ij}) for u_i with Hessian H_i (constant)."""
+    rng = np.random.default_rng(1)
+    D = len(dims)
+    μ = 0.05
+    o = _sim(dims, ν=μ, λμ=1.0, λρ=1.0)
+    Ng = o.Ng
+    o.set(O.F, np.asfortranarray(rng.uniform(0, 1, Ng)))  # any f: single phase
+    H = [None] * D
+    u = np.zeros(Ng + (D,), order="F")
+    for i in range(D):
+        A = rng.uniform(-1, 1, (D, D))
+        H[i] = (A + A.T) / 10  # Hessian of u_i
+        b = rng.uniform(-1, 1, D)
+        X = _face_coords(Ng, i)
+        val = np.full(Ng, 0.3)
+        for a in range(D):
+            val += b[a] * X[a]
+            for c in range(D):
+                val += 0.5 * H[i][a, c] * X[a] * X[c]
+        u[..., i] = val
+    o.set(O.U, u)
+    o.lib.oc_visc(o.ctx, 0)
+    r = o.get(O.FORCE)
+    inner = tuple(slice(1, n + 1) for n in dims)
+    for i in range(D):
+        expect = μ * (np.trace(H[i]) + sum(H[j][i, j] for j in range(D)))
+        got = r[inner + (i,)]
+        assert np.abs(got - expect).max() <= 1e-11 * max(1.0, abs(expect)), (i, expect, got.min(), got.max())
+
+
+@pytest.mark.parametrize("mode", ["shear", "compression"])
+def test_visc_periodic_fourier_modes(mode):
+    """Periodic in x and y (2D, N = 16): u_x = cos(k y) (shear) gives
+    μ(2cos k - 2) cos(k y); u_x = cos(k x) (compression, ∇·u ≠ 0) gives twice
+    that, through the transposed gradient."""
+    N = 16
+    μ = 0.2
+    o = _sim((N, N), ν=μ, λμ=1.0, λρ=1.0, perdir=(1, 2))
+    Ng = o.Ng
+    o.set(O.F, np.ones(Ng, order="F"))
+    k = 2 * np.pi / N
+    X, Y = _face_coords(Ng, 0)
+    u = np.zeros(Ng + (2,), order="F")
+    u[..., 0] = np.cos(k * Y) if mode == "shear" else np.cos(k * X)
+    o.set(O.U, u)
+    o.lib.oc_visc(o.ctx, 0)
+    r = o.get(O.FORCE)
+    lam = 2 * np.cos(k) - 2
+    inner = (slice(1, N + 1), slice(1, N + 1))
+    expect = μ * lam * (np.cos(k * Y) if mode == "shear" else 2 * np.cos(k * X))
+    assert np.abs(r[inner + (0,)] - expect[inner]).max() <= 1e-12
+    assert np.abs(r[inner + (1,)]).max() <= 1e-12
+
+
+# ----------------------------------------------------------------------------- MPCFL
+def _cfl_case(u_scale=1.0, **kw):
+    """2D 8x8 walls; cell (2,2) (0-based, ghosts included) has u_x = 0.5 on its
+    upper x face and u_y = -0.25 on its upper y face, every other face at rest.
+    By hand: maxTotalFlux is 0.75 at cell (2,2) (0.5 + 0.25), 0.5 at (3,2) and
+    0.25 at (2,3); flux_out is 0.5 at (2,2) (through the x face), 0 at (3,2)
+    (inflow) and 0.25 at (2,3) (the downward flow leaves it)."""
+    o = _sim((8, 8), **kw)
+    u = np.zeros(o.Ng + (2,), order="F")
+    u[3, 2, 0] = 0.5 * u_scale
+    u[2, 3, 1] = -0.25 * u_scale
+    o.set(O.U, u)
+    return o.cfl()
+
+
+def test_mpcfl_each_limit_active():
+    # Δt_cVOF = 1/(2·0.75) is the smallest: 0.8/1.5
+    assert _cfl_case() == pytest.approx(0.8 / 1.5, rel=1e-14)
+    # Δt_Adv = 1/(0.5 + 5ν) = 0.5 with ν = 0.3, λμ = λρ (Δt_Visc = 3/4.2 = 0.714)
+    assert _cfl_case(ν=0.3, λμ=1.0, λρ=1.0) == pytest.approx(0.8 * 0.5, rel=1e-14)
+    # Δt_Grav = 1/(2|g|) = 0.25 with g = (0, -2)
+    assert _cfl_case(g=(0.0, -2.0)) == pytest.approx(0.8 * 0.25, rel=1e-14)
+    # |g| = √(3² + 4²) = 5: 0.1
+    assert _cfl_case(g=(3.0, -4.0)) == pytest.approx(0.8 * 0.1, rel=1e-14)
+    # Δt_Visc = 3/(14 ν max(1, λμ/λρ)) = 3/7 with ν = 0.05, λμ/λρ = 10 (Δt_Adv = 1/0.75)
+    assert _cfl_case(ν=0.05, λμ=1e-2, λρ=1e-3) == pytest.approx(0.8 * 3 / 7, rel=1e-14)
+    # ... and max(1, λμ/λρ) = 1 when λμ < λρ: ν = 0.3, u scaled by 1/10: Δt_Visc = 3/4.2 = 0.714
+    # (it would be 0.0714 with the ratio inverted), Δt_Adv = 1/(0.05 + 1.5) = 0.645,
+    # Δt_cVOF = 1/0.15 = 6.667: Adv is active
+    assert _cfl_case(u_scale=0.1, ν=0.3, λμ=1e-3, λρ=1e-2) == pytest.approx(0.8 / 1.55, rel=1e-14)
+    # Δt_SurfT = √((1 + λρ)/(8π η)) with η = 1, λρ = 1e-3
+    assert _cfl_case(η=1.0, λρ=1e-3) == pytest.approx(0.8 * np.sqrt(1.001 / (8 * np.pi)), rel=1e-14)
+    # nothing moves, no physics: every limit is Δt_max = 1 or infinite
+    o = _sim((8, 8))
+    assert o.cfl() == pytest.approx(0.8, rel=1e-14)
+
+
+# ----------------------------------------------------------------------------- ϕq
+def _phiq(Psi, fd, dt, lr, u, c, d, lim="Koren"):
+    return O.lib(T).oc_phiq_value(Psi, fd, dt, lr, O.LIMITER[lim], u, c, d)
+
+
+def test_phiq_regimes():
+    # Koren(0, 1, 3) = median((7 + 3)/6, 1, median(2, 1, 3)) = 5/3, so va = 2 - 5/3 = 1/3
+    # donor regime: |Ψ|δt = 2 > mOld = lin(0, 0.5) = 0.5 -> Ψ·c = 2
+    assert _phiq(2.0, 0.0, 1.0, 0.5, 0.0, 1.0, 3.0) == 2.0
+    # the same with Ψ < 0: -2·c
+    assert _phiq(-2.0, 0.0, 1.0, 0.5, 0.0, 1.0, 3.0) == -2.0
+    # blend: Ψ = 0.5, δt = 1, fd = 1 -> mOld = 1, l2 = 0.5, vb = (1/3 + 5/3)/2 = 1,
+    # Ψ(vb + vd)/2 = 0.5 (1 + 5/3)/2 = 2/3
+    assert _phiq(0.5, 1.0, 1.0, 0.5, 0.0, 1.0, 3.0) == pytest.approx(2 / 3, rel=1e-15)
+    # blend with a light donor: fd = 0.2, λρ = 0.5 -> mOld = 0.6; Ψ = 0.3, δt = 1 -> l2 = 1/2 again
+    assert _phiq(0.3, 0.2, 1.0, 0.5, 0.0, 1.0, 3.0) == pytest.approx(0.3 * (1 + 5 / 3) / 2, rel=1e-14)
+    # l2 -> 0: Ψ times the limiter's face value 5/3
+    v = _phiq(1e-9, 1.0, 1.0, 0.5, 0.0, 1.0, 3.0)
+    assert v / 1e-9 == pytest.approx(5 / 3, rel=1e-8)
+    # minmod(0, 1, 3) = median(3/2, 1, 2) = 3/2, l2 = 1/4: vb = (1/4)(1/2) + (3/4)(3/2) = 5/4,
+    # Ψ(vb + vd)/2 = 0.25 (5/4 + 3/2)/2 = 11/32
+    assert _phiq(0.25, 1.0, 1.0, 0.5, 0.0, 1.0, 3.0, "minmod") == pytest.approx(11 / 32, rel=1e-15)
+    # mOut == mOld is not the donor branch (strict >): l2 = 1 -> Ψ(va + vd)/2 = Ψ·c
+    assert _phiq(1.0, 1.0, 1.0, 0.5, 0.0, 1.0, 3.0) == pytest.approx(1.0, rel=1e-15)
+
+
+# ----------------------------------------------------------------------------- gravity
+@pytest.mark.parametrize("yint", [8.0, 8.3])
+def test_flat_interface_under_gravity_is_hydrostatic(yint):
+    N = 16
+    g = -0.05
+    o = _sim((N, N), λρ=1e-3, λμ=1e-2, g=(0.0, g), InterfaceSDF=lambda x: x[1] - yint, poisson="refined",
+             poisson_rtol=1e-12)
+    f0 = o.f
+    for _ in range(2):
+        o.sim_step()
+    assert np.abs(o.f - f0).max() <= 1e-12  # nothing moves (u is 0 to the solver's tolerance)
+    assert np.abs(o.u).max() <= 1e-10 * abs(g) * float(o.Δt[-2])
+    p = o.p
+    L = o.get(O.MU0)
+    # every vertical face between interior cells: p_j - p_{j-1} = g / L_y = ρ_face g
+    dp = p[1:N + 1, 2:N + 1] - p[1:N + 1, 1:N]
+    expect = g / L[1:N + 1, 2:N + 1, 1]
+    assert np.abs(dp - expect).max() <= 1e-9 * abs(g)
+    # and ρ_face is the face-averaged density of updateL!: 1 in the liquid, λρ in the gas
+    assert expect.min() == pytest.approx(g, rel=1e-12)
+    assert expect.max() == pytest.approx(g * 1e-3, rel=1e-12)
+
+
+def test_periodic_free_fall():
+    """Periodic in x and y, two phases at rest with a flat interface, gravity
+    -y: nothing resists the fall, so one step gives u = g Δt everywhere
+    (exactly: uniform u is divergence-free and CMOM keeps it uniform), and the
+    corrector advects f with the predictor's u = g Δt/2, which translates the
+    planar interface by g Δt²/2 (exact for a planar interface in its cell)."""
+    N = 16
+    g = -0.05
+    dt = 0.5
+    o = _sim((N, N), λρ=1e-3, λμ=1e-2, g=(0.0, g), InterfaceSDF=lambda x: x[1] - 8.3, perdir=(1, 2), Δt=dt)
+    assert float(o.Δt[-1]) == dt
+    f0 = o.f
+    o.sim_step()
+    u = o.u
+    assert np.abs(u[..., 1] - g * dt).max() <= 1e-14
+    assert np.abs(u[..., 0]).max() <= 1e-14
+    f1 = o.f
+    rows = [j for j in range(1, N + 1) if 0 < f0[5, j] < 1]
+    assert len(rows) == 1
+    j = rows[0]
+    assert np.abs(f1[1:N + 1, j] - (f0[1:N + 1, j] + g * dt * dt / 2)).max() <= 1e-14
+    # the periodic wrap is a second, cell-aligned interface (liquid row 1 under
+    # gas row N): the top row receives the fluid leaving row 1 downwards
+    assert np.abs(f1[1:N + 1, N] - (-g * dt * dt / 2)).max() <= 1e-14
+    others = [q for q in range(1, N) if q != j]
+    assert np.array_equal(f1[1:N + 1, others], f0[1:N + 1, others])
+
+
+# ----------------------------------------------------------------------------- reportFillError
+def _squeeze(U, dt):
+    """A column of f = 0.4 between full columns, squeezed by u_x = ±U (u⁰ = u)
+    from both sides: δl = U δt of fluid enters through each face."""
+    N = 16
+    o = _sim((N, N), perdir=(1, 2))
+    f = np.ones(o.Ng, order="F")
+    f[8, :] = 0.4
+    o.set(O.F, f)
+    u = np.zeros(o.Ng + (2,), order="F")
+    u[8, :, 0] = U    # lower x face of column 8: inflow
+    u[9, :, 0] = -U   # upper x face: inflow
+    o.set(O.U, u)
+    o.set(O.U0, u)
+    return o.advect(dt, 1)
+
+
+def test_divergent_flow_at_the_interface_explodes():
+    """δl = 0.4 from each side takes f to 1.2.  |∇·u⁰| + |∇·u| is 4U there:
+    U = 3 gives 12 > 10, the explosion (advection.jl:160, status 2); U = 2.5
+    gives exactly 10, only the printed fill error (status 3); a tenth of the
+    inflow stays in [0, 1]."""
+    assert _squeeze(3.0, 0.4 / 3.0) == 2
+    assert _squeeze(2.5, 0.4 / 2.5) == 3
+    assert _squeeze(3.0, 0.04 / 3.0) == 0
