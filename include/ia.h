@@ -191,6 +191,17 @@ int ia_project(ia_ctx *c, double dt_w, ia_solve_stats *stats);
  * be NULL (zero). */
 int ia_metrics(ia_ctx *c, const double *U, const double *statwl, double *out);
 
+/* ia_metrics without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight time-stepping or serving loops, harnesses that forbid hipMalloc).
+ * Same integrals as ia_metrics, summed in another fixed order (equal to
+ * round-off).  Its partial sums live in a static device array of the library
+ * (1024 block partials x 7 + 8 doubles, 57 KiB per GPU, held until the
+ * process ends); any grid size folds into those 1024 partials.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (two threads, two
+ * contexts on one device) race; serialise them. */
+int ia_metrics_noalloc(ia_ctx *c, const double *U, const double *statwl, double *out);
+
 /* LevelSet(sim)  redistaning.jl:14-25: ϕ = 2f - 1 on every entry of cVOF.f⁰
  * (IA_FIELD_F0 then reads ϕ) and ϕini = ϕ in a scratch buffer of the context
  * (the reference's cVOF.fᶠ), valid until f is next advected or re-applied. */

@@ -90,6 +90,7 @@ def lib():
         "ia_profile_read": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "ia_last_error": (C.c_char_p, [vp]),
         "ia_metrics": (C.c_int, [vp, vp, vp, vp]),
+        "ia_metrics_noalloc": (C.c_int, [vp, vp, vp, vp]),
         "ia_levelset": (C.c_int, [vp]),
         "ia_redistance": (C.c_int, [vp, C.c_double, C.c_double, C.c_int]),
         "ia_learned_init": (C.c_int, [vp, C.c_uint]),

@@ -17,9 +17,43 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 using namespace ia;
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free diagnostics entry point (ia_metrics_noalloc):
+// the per-block partial sums of the metrics integrals, then the 7 totals.
+// Every GPU the library runs on holds one copy until the process ends; in
+// host code `scratch` is only a symbol, and its device address comes from
+// hipGetSymbolAddress, once per device, at context creation (noalloc_ptr).
+// ---------------------------------------------------------------------------
+#define IA_NOALLOC_BLOCKS 1024  // partial sums: at most this many blocks of NT threads
+#define IA_NOALLOC_DOUBLES (IA_NOALLOC_BLOCKS * 7 + 8)
+static __device__ __align__(256) double scratch[IA_NOALLOC_DOUBLES];
+namespace {
+constexpr int NOALLOC_MAXDEV = 64;
+double *noalloc_dptr[NOALLOC_MAXDEV] = {};
+std::mutex noalloc_mu;
+// the current device's address of `scratch` (the caller has made c->dev current)
+int noalloc_setup(int dev, const char **why) {
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) {
+        *why = "device ordinal out of range for the no-allocation scratch";
+        return IA_ERR_BADARG;
+    }
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_dptr[dev]) return IA_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        *why = hipGetErrorString(e);
+        return IA_ERR_HIP;
+    }
+    noalloc_dptr[dev] = (double *)ptr;
+    return IA_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -244,6 +278,43 @@ __global__ void __launch_bounds__(NT) k_bcvel(Grid g, Tiling t, T *a, T d0, T d1
     long I = lin(g, p);
     T dv[3] = {d0, d1, d2};
     for (int i = 0; i < g.D; ++i) write_images<T>(g, a + g.ns * i, p, a[g.ns * i + I], 1, i, dv[i]);
+}
+
+// metrics integrands, grid-stride over the tiles: every block folds the tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... in that order, then writes its 7
+// block sums (ia_metrics_noalloc; part = the scratch array's partials)
+template <int D, class T>
+__global__ void __launch_bounds__(NT) k_metrics_gs(Grid g, Tiling t, const T *u, const T *f, T lr, double U0,
+                                                   double U1, double U2, double g0, double g1, double g2, double w0,
+                                                   double w1, double w2, double *part) {
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    double U[3] = {U0, U1, U2}, gv[3] = {g0, g1, g2}, wl[3] = {w0, w1, w2};
+    int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    for (long b = blockIdx.x; b < t.nblk; b += gridDim.x) {
+        long bx = b % t.nt[0], r = b / t.nt[0], by = r % t.nt[1], bz = r / t.nt[1];
+        int p[3];
+        p[0] = 1 + (int)bx * TX + tx;
+        p[1] = g.D > 1 ? 1 + (int)by * TY + ty : 0;
+        p[2] = g.D > 2 ? 1 + (int)bz : 0;
+        bool ok = p[0] <= g.Ng[0] - 2;
+        if (g.D > 1) ok = ok && p[1] <= g.Ng[1] - 2;
+        if (g.D > 2) ok = ok && p[2] <= g.Ng[2] - 2;
+        if (ok) metrics_cell<D, T>(g, u, f, lr, U, gv, wl, p, acc);
+    }
+    for (int q = 0; q < 7; ++q) {
+        double s = block_sum(acc[q]);
+        if (threadIdx.x == 0) part[(long)blockIdx.x * 7 + q] = s;
+    }
+}
+// the 7 totals of nb block partials (each thread adds its blocks in order,
+// then a fixed tree)
+__global__ void __launch_bounds__(NT) k_metrics_fin(const double *part, long nb, double *out) {
+    for (int q = 0; q < 7; ++q) {
+        double s = 0;
+        for (long b = threadIdx.x; b < nb; b += NT) s += part[b * 7 + q];
+        s = block_sum(s);
+        if (threadIdx.x == 0) out[q] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1079,6 +1150,27 @@ template <class T> struct Impl {
         }
         return IA_OK;
     }
+    // ia_metrics without allocating: block partials and totals in the
+    // device's `scratch` (address ws); the totals are copied into `out`
+    static int metrics_noalloc(ia_ctx *c, double *ws, const double *U, const double *wl, double *out) {
+        Grid &g = c->g;
+        long nb = std::min<long>(c->t.nblk, IA_NOALLOC_BLOCKS);
+        if (nb * 7 + 7 > IA_NOALLOC_DOUBLES)
+            return fail(c, IA_ERR_BADARG, "ia_metrics_noalloc: %ld block partials exceed the scratch of %d doubles",
+                        nb, IA_NOALLOC_DOUBLES);
+        double gv[3] = {0, 0, 0};
+        if (c->P.has_g)
+            for (int d = 0; d < g.D; ++d) gv[d] = (double)(T)c->P.g[d];
+        double *tot = ws + (long)IA_NOALLOC_BLOCKS * 7;
+        auto km = g.D == 3 ? k_metrics_gs<3, T> : k_metrics_gs<2, T>;
+        hipLaunchKernelGGL(km, dim3((unsigned)nb), dim3(NT), 0, c->s, g, c->t, P<T>(c->u), P<T>(c->fb[c->fi]),
+                           (T)c->P.lambda_rho, U[0], U[1], U[2], gv[0], gv[1], gv[2], wl[0], wl[1], wl[2], ws);
+        hipLaunchKernelGGL(k_metrics_fin, dim3(1), dim3(NT), 0, c->s, ws, nb, tot);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(out, tot, 7 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return IA_OK;
+    }
     // ----------------------------------------------------------------- sweeps
     static void prepare(ia_ctx *c, const void *f, const void *usrc, bool do_ru) {
         hipLaunchKernelGGL(k_prepare<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(f), P<T>(usrc), P<T>(c->ru),
@@ -1473,7 +1565,10 @@ int ia_create(const ia_params *p, ia_ctx **out) {
         delete c;
         return IA_ERR_HIP;
     }
-    int r = dispatch(c, [&](auto impl) { return decltype(impl)::create(c); });
+    const char *why = "";
+    int r = noalloc_setup(c->dev, &why);
+    if (r != IA_OK) fail(c, r, "no-allocation scratch: %s", why);
+    if (r == IA_OK) r = dispatch(c, [&](auto impl) { return decltype(impl)::create(c); });
     if (r == IA_OK) r = hipStreamSynchronize(c->s) == hipSuccess ? IA_OK : IA_ERR_HIP;
     if (r != IA_OK) {
         fprintf(stderr, "ia_create: %s\n", c->msg);
@@ -1627,6 +1722,16 @@ int ia_metrics(ia_ctx *c, const double *U, const double *statwl, double *out) {
     double z[3] = {0, 0, 0};
     (void)hipSetDevice(c->dev);
     return dispatch(c, [&](auto impl) { return decltype(impl)::metrics(c, U ? U : z, statwl ? statwl : z, out); });
+}
+int ia_metrics_noalloc(ia_ctx *c, const double *U, const double *statwl, double *out) {
+    if (!c || !out) return IA_ERR_BADARG;
+    if (c->dev < 0 || c->dev >= NOALLOC_MAXDEV || !noalloc_dptr[c->dev])
+        return fail(c, IA_ERR_BADARG, "ia_metrics_noalloc: no scratch address for device %d", c->dev);
+    double *ws = noalloc_dptr[c->dev];
+    double z[3] = {0, 0, 0};
+    return dispatch(c, [&](auto impl) {
+        return decltype(impl)::metrics_noalloc(c, ws, U ? U : z, statwl ? statwl : z, out);
+    });
 }
 int ia_levelset(ia_ctx *c) {
     if (!c) return IA_ERR_BADARG;
