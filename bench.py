@@ -75,19 +75,61 @@ PROFILED = ["load_inc", "prolong_inc", "cheb2_dots", "fcg_Ap", "fcg_update", "vo
 
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs (copy-calibrated), when such a measurement is committed."""
+    WRITE_SIZE runs (copy-calibrated), when such a measurement is committed,
+    and where it comes from (file, commands, the commit it was measured at)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(kernel)
+            d = json.load(fh)
     except (OSError, ValueError):
-        return None
+        return None, None
+    src = d.get("_source", {})
+    return d.get(kernel), dict(file="profiles/pmc_traffic.json", measured_at_commit=src.get("commit"),
+                               commands=src.get("commands"), calibration=d.get("_calib", {}).get("how"))
 
 
-def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
+def error_vs_fp64_oracle(ia, O, N=64, steps=5, report=(1, 5)):
+    """The benchmarked FP32 step against the FP64 oracle from identical inputs:
+    the pool case at N^3 (the same generator: 32 bubbles, seed 3), f from the
+    device's applyVOF! handed to the FP64 oracle, both runs on the device's
+    Δt sequence.  Field errors after the steps in `report`."""
+    bl, kw = make_case(N, "refined")
+    g, _, _ = build_sim(ia, N, "refined", 1e-6)
+    okw = dict(kw)
+    okw["T"] = np.float64
+    o = O.OracleSim((N,) * 3, (0.0, 0.0, 0.0), float(N), poisson="refined", poisson_rtol=1e-6, **okw)
+    o.set(O.F, g.intf.f.astype(np.float64))
+    o.set(O.F0, g.intf.f0.astype(np.float64))
+    o.Δt = [np.float64(g.flow.Δt[-1])]
+    V0 = float(O.inside(g.intf.f).astype(np.float64).sum())
+    out = {}
+    for n in range(1, steps + 1):
+        st = ia.sim_step_(g)
+        o.sim_step()
+        o.Δt[-1] = np.float64(g.flow.Δt[-1])
+        if n in report:
+            fg, fo = g.intf.f.astype(np.float64), o.f
+            ug, uo = g.flow.u.astype(np.float64), o.u
+            pg, po = g.flow.p.astype(np.float64), o.p
+            out[f"step_{n}"] = dict(
+                f_max_abs=float(np.abs(fg - fo).max()),
+                u_max_abs_rel=float(np.abs(ug - uo).max() / max(np.abs(uo).max(), 1e-30)),
+                p_max_abs_rel=float(np.abs(O.inside(pg - pg.mean()) - O.inside(po - po.mean())).max()
+                                    / max(np.abs(O.inside(po - po.mean())).max(), 1e-30)),
+                volume_drift_rel=float((st.vol - V0) / V0),
+                f_range=[st.fmin, st.fmax],
+                solve_rel_residual=[st.solve[0].rel, st.solve[1].rel])
+    g.close()
+    out["case"] = (f"pool {N}^3 (32 bubbles, seed 3), FP32 device (REFINED, 1e-6) vs the FP64 oracle from the "
+                   f"same f, same Δt sequence")
+    return out
+
+
+def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1, ia=None):
     """The CPU oracle (C + OpenMP, same algorithm: FP32 stencils, FP64 flexible CG
     around the FP32 V-cycle to the same relative residual) on the same case at
-    a bounded size, timed after `warmup` untimed steps."""
+    a bounded size, timed after `warmup` untimed steps.  With `ia`, the same leg
+    also reports the device step's error against the FP64 oracle at 64^3."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     bl, kw = make_case(N_cpu, poisson)
@@ -105,10 +147,13 @@ def cpu_baseline(N_cpu, poisson, rtol, steps, warmup=1):
         o.sim_step()
         its += [o.lib.oc_last_iters(o.ctx, 0), o.lib.oc_last_iters(o.ctx, 1)]
     t = time.perf_counter() - t0
-    return dict(value=round(N_cpu ** 3 * steps / t / 1e6, 4), unit="Mcell-steps/s", cores=threads, kind="port",
-                sample=f"CPU oracle (C, OpenMP, FP32 stencils, FP64 flexible CG + FP32 V-cycle to {rtol:g}), "
-                       f"the same pool+bubbles case at {N_cpu}^3, {steps} step(s) after {warmup} warm-up, "
-                       f"{t:.1f} s, CG iterations per solve {its}")
+    rec = dict(value=round(N_cpu ** 3 * steps / t / 1e6, 4), unit="Mcell-steps/s", cores=threads, kind="port",
+               sample=f"CPU oracle (C, OpenMP, FP32 stencils, FP64 flexible CG + FP32 V-cycle to {rtol:g}), "
+                      f"the same pool+bubbles case at {N_cpu}^3, {steps} step(s) after {warmup} warm-up, "
+                      f"{t:.1f} s, CG iterations per solve {its}")
+    if ia is not None:
+        rec["error_vs_fp64_oracle"] = error_vs_fp64_oracle(ia, O)
+    return rec
 
 
 def poisson_compare(ia, N, rtol, warm=8, steps=3, lr=1e-3, case=None):
@@ -278,12 +323,14 @@ def main():
                               f"(the roofline kernel: the timed steps)",
     }
     if dom:
+        traffic, traffic_src = pmc_traffic(dom) if (args.case, N) == ("pool", 256) else (None, None)
         avg_ms = timed[dom]["avg_ms"]
         bpl = BYTES_PER_CELL[dom] * timed[dom]["cells"]
         achieved = bpl / (avg_ms * 1e-3) / 1e9
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
-                           "traffic": pmc_traffic(dom) if (args.case, N) == ("pool", 256) else None,
+                           "traffic": traffic,
+                           "traffic_source": traffic_src,
                            "bytes_per_launch": int(bpl), "avg_launch_ms": avg_ms,
                            "launches": timed[dom]["launches"]}
     out["levelset_redistance"] = redistance_rate(ia, sim, cells)
@@ -292,7 +339,8 @@ def main():
         out["config"]["poisson_compare"] = poisson_compare(ia, N, args.rtol, warm=args.compare_warm,
                                                            steps=args.compare_steps, case=case)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_N, args.poisson, args.rtol, args.cpu_steps, ia=ia)
+        out["error_vs_fp64_oracle"] = out["cpu_baseline"].pop("error_vs_fp64_oracle")
     print(json.dumps(out), flush=True)
 
 

@@ -198,6 +198,44 @@ def test_advect_single_vortex(T):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_divergent_squeeze_explodes(T):
+    """reportFillError's explosion (advection.jl:160): a column of f = 0.4
+    between full columns, squeezed by u_x = ±U from both sides with δl = 0.4.
+    f reaches 1.2 where |∇·u⁰| + |∇·u| = 4U: U = 3 (12 > 10) raises the
+    divergence error (status 2), U = 2.5 (exactly 10) only warns (status 3),
+    as on the oracle."""
+    N = 16
+    for U, want in ((3.0, 2), (2.5, 3)):
+        say("explosion squeeze", T.__name__, U)
+        sim = ia.TwoPhaseSimulation((N, N), (0.0, 0.0), float(N), T=T, perdir=(1, 2))
+        o = O.OracleSim((N, N), (0.0, 0.0), float(N), T=T, perdir=(1, 2))
+        f = np.ones(sim.Ng, dtype=T, order="F")
+        f[8, :] = 0.4
+        u = np.zeros(sim.Ng + (2,), dtype=T, order="F")
+        u[8, :, 0] = U
+        u[9, :, 0] = -U
+        for a in (sim, ):
+            a.intf.f = f
+            a.flow.u = u
+            a.flow.u0 = u
+        o.set(O.F, f)
+        o.set(O.U, u)
+        o.set(O.U0, u)
+        dt = 0.4 / U
+        assert o.advect(dt, 1) == want
+        if want == 2:
+            with pytest.raises(ia.IAError) as e:
+                ia.advect_(sim, dt=dt)
+            assert e.value.code == 2 and "exploding" in str(e.value)
+        else:
+            st = ia.advect_(sim, dt=dt)
+            assert st.status == 3 and st.fill_warnings == 1
+            assert maxdiff(sim.intf.f, o.f) <= (1e-12 if T == np.float64 else 1e-6)
+        sim.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_advect_named_fields(T):
     """advect!(a, c, f, u¹, u², dt) with f = c.f⁰ and the velocities taken from
     Flow.f and cVOF.ρu (advection.jl:17): the same sweeps as the oracle's
