@@ -10,12 +10,14 @@ reproduces the run bit for bit.
 VTK files hold every array entry, ghosts included, as point data on a grid of
 Ng points per axis (extent 0..Ng-1), vectors with their components first, in
 base64-encoded little-endian binary with a UInt64 byte-count header -- the
-layout ParaView and ReadVTK.jl read.
+layout ParaView and ReadVTK.jl read.  The reader also takes appended raw data
+and zlib-compressed blocks, the layouts other VTK writers use.
 """
 import base64
 import json
 import os
 import xml.etree.ElementTree as ET
+import zlib
 
 import numpy as np
 
@@ -77,12 +79,57 @@ def write_vti(path, fields, Ng):
     ET.ElementTree(root).write(path, xml_declaration=True, encoding="utf-8")
 
 
+def _blocks(buf, pos, hdr, compressed):
+    """One DataArray's bytes starting at `pos` of `buf`: a byte-count header
+    then the raw bytes, or with zlib compression the block table
+    [nblocks, block size, last block size, compressed sizes...] then the
+    compressed blocks (the VTK XML binary layout)."""
+    hs = np.dtype(hdr).itemsize
+    if not compressed:
+        n = int(np.frombuffer(buf[pos:pos + hs], dtype=hdr)[0])
+        return buf[pos + hs:pos + hs + n]
+    nb = int(np.frombuffer(buf[pos:pos + hs], dtype=hdr)[0])
+    table = np.frombuffer(buf[pos:pos + (3 + nb) * hs], dtype=hdr).astype(np.int64)
+    q = pos + (3 + nb) * hs
+    out = []
+    for k in range(nb):
+        c = int(table[3 + k])
+        out.append(zlib.decompress(buf[q:q + c]))
+        q += c
+    return b"".join(out)
+
+
+def _inline_blocks(text, hdr, compressed):
+    """An inline base64 DataArray: uncompressed, one base64 stream of header +
+    bytes; zlib-compressed, the block table and the blocks as two streams."""
+    if not compressed:
+        return _blocks(base64.b64decode(text), 0, hdr, False)
+    hs = np.dtype(hdr).itemsize
+    first = base64.b64decode(text[:4 * ((hs + 2) // 3)])
+    nb = int(np.frombuffer(first[:hs], dtype=hdr)[0])
+    hl = 4 * (((3 + nb) * hs + 2) // 3)
+    table = base64.b64decode(text[:hl])
+    return _blocks(table + base64.b64decode(text[hl:]), 0, hdr, True)
+
+
 def read_vti(path):
-    """Read a file written by `write_vti` (or any uncompressed binary ImageData
-    with a UInt64 header): returns ({name: array}, Ng)."""
-    root = ET.parse(path).getroot()
-    if root.get("compressor"):
-        raise ValueError("compressed VTK files are not supported")
+    """Read a VTK ImageData file: binary DataArrays inline (base64) or appended
+    (raw, after the AppendedData marker), uncompressed or zlib-compressed
+    (vtkZLibDataCompressor), UInt32 or UInt64 headers -- what `write_vti`
+    writes and what WriteVTK-based writers write by default.  Returns
+    ({name: array}, Ng)."""
+    data = open(path, "rb").read()
+    raw = None
+    i = data.find(b"<AppendedData")
+    if i >= 0:
+        j = data.index(b"_", data.index(b">", i)) + 1
+        raw = data[j:]
+        data = data[:i] + b"</VTKFile>"
+    root = ET.fromstring(data)
+    comp = root.get("compressor")
+    if comp not in (None, "", "vtkZLibDataCompressor"):
+        raise ValueError(f"unsupported VTK compressor {comp}")
+    compressed = bool(comp)
     hdr = np.uint64 if root.get("header_type", "UInt32") == "UInt64" else np.uint32
     img = root.find("ImageData")
     e = [int(v) for v in img.get("WholeExtent").split()]
@@ -91,12 +138,17 @@ def read_vti(path):
     Ng = Ng[:D]
     out = {}
     for da in img.find("Piece").find("PointData").findall("DataArray"):
-        if da.get("format") != "binary":
-            raise ValueError("only binary DataArrays are supported")
-        buf = base64.b64decode(da.text.strip())
-        n = int(np.frombuffer(buf[:np.dtype(hdr).itemsize], dtype=hdr)[0])
+        fmt = da.get("format")
+        if fmt == "binary":
+            buf = _inline_blocks(da.text.strip(), hdr, compressed)
+        elif fmt == "appended":
+            if raw is None:
+                raise ValueError("appended DataArray without AppendedData")
+            buf = _blocks(raw, int(da.get("offset")), hdr, compressed)
+        else:
+            raise ValueError(f"DataArray format {fmt!r} is not supported (binary or appended)")
         dt = np.dtype(_NP_TYPE[da.get("type")]).newbyteorder("<")
-        flat = np.frombuffer(buf[np.dtype(hdr).itemsize:np.dtype(hdr).itemsize + n], dtype=dt)
+        flat = np.frombuffer(buf, dtype=dt)
         nc = int(da.get("NumberOfComponents", "1"))
         if nc == 1:
             out[da.get("Name")] = flat.reshape(Ng, order="F").astype(dt.newbyteorder("="))

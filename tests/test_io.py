@@ -35,6 +35,79 @@ def test_vti_round_trip(tmp_path, dt, D):
     assert (first == v[(0,) * D]).all()
 
 
+def _hand_vti(path, arrays, Ng, fmt, zlib_blocks, hdr):
+    """A VTK ImageData file written by hand, the way other VTK writers lay it
+    out: format "binary" (inline base64) or "appended" (raw bytes after '_'),
+    optionally vtkZLibDataCompressor blocks of `zlib_blocks` bytes, with UInt32
+    or UInt64 headers."""
+    import base64
+    import zlib
+    hs = np.dtype(hdr).itemsize
+
+    def encode(raw):
+        if zlib_blocks is None:
+            return [np.array([len(raw)], dtype=hdr).tobytes() + raw]
+        chunks = [raw[i:i + zlib_blocks] for i in range(0, len(raw), zlib_blocks)]
+        comp = [zlib.compress(c) for c in chunks]
+        table = np.array([len(chunks), zlib_blocks, len(chunks[-1])] + [len(c) for c in comp], dtype=hdr)
+        return [table.tobytes(), b"".join(comp)]
+
+    ext = " ".join(f"0 {Ng[a] - 1}" if a < len(Ng) else "0 0" for a in range(3))
+    comp = ' compressor="vtkZLibDataCompressor"' if zlib_blocks else ""
+    head = (f'<?xml version="1.0"?>\n<VTKFile type="ImageData" version="1.0" byte_order="LittleEndian" '
+            f'header_type="{"UInt64" if hs == 8 else "UInt32"}"{comp}>\n'
+            f'<ImageData WholeExtent="{ext}" Origin="0 0 0" Spacing="1 1 1">\n<Piece Extent="{ext}">\n<PointData>\n')
+    body, appended = "", b""
+    for name, a in arrays.items():
+        nc = 1 if a.shape == tuple(Ng) else a.shape[-1]
+        flat = np.moveaxis(a, -1, 0) if nc > 1 else a
+        raw = np.asfortranarray(flat).tobytes(order="F")
+        parts = encode(raw)
+        t = {np.float32: "Float32", np.float64: "Float64"}[a.dtype.type]
+        if fmt == "binary":
+            text = "".join(base64.b64encode(p).decode() for p in parts)
+            body += (f'<DataArray type="{t}" Name="{name}" NumberOfComponents="{nc}" format="binary">'
+                     f'{text}</DataArray>\n')
+        else:
+            body += (f'<DataArray type="{t}" Name="{name}" NumberOfComponents="{nc}" format="appended" '
+                     f'offset="{len(appended)}"/>\n')
+            appended += b"".join(parts)
+    tail = "</PointData>\n</Piece>\n</ImageData>\n"
+    with open(path, "wb") as fh:
+        fh.write((head + body + tail).encode())
+        if fmt == "appended":
+            fh.write(b'<AppendedData encoding="raw">\n_' + appended + b"\n</AppendedData>\n")
+        fh.write(b"</VTKFile>\n")
+
+
+@pytest.mark.parametrize("fmt,blocks,hdr", [("appended", None, np.uint64), ("appended", 96, np.uint32),
+                                            ("binary", 96, np.uint64), ("binary", 1 << 15, np.uint32),
+                                            ("binary", None, np.uint32)])
+def test_read_vti_other_layouts(tmp_path, fmt, blocks, hdr):
+    """Appended raw data (with '_' and bytes that look like XML) and zlib
+    blocks, several blocks with a short last one, both header widths."""
+    rng = np.random.default_rng(7)
+    Ng = (7, 5, 3)
+    s = np.asfortranarray(rng.standard_normal(Ng))
+    s.flat[3] = np.frombuffer(b"<_>\n</Ap", dtype=np.float64)[0]
+    v = np.asfortranarray(rng.standard_normal(Ng + (3,)).astype(np.float32))
+    path = os.path.join(tmp_path, "o.vti")
+    _hand_vti(path, {"p": s, "u": v}, Ng, fmt, blocks, hdr)
+    got, Ng2 = io.read_vti(path)
+    assert tuple(Ng2) == Ng
+    assert got["p"].dtype == np.float64 and np.array_equal(got["p"], s)
+    assert got["u"].dtype == np.float32 and np.array_equal(got["u"], v)
+
+
+def test_read_vti_rejects_other_compressors(tmp_path):
+    path = os.path.join(tmp_path, "c.vti")
+    _hand_vti(path, {"p": np.zeros((3, 3))}, (3, 3), "binary", 64, np.uint64)
+    txt = open(path, "rb").read().replace(b"vtkZLibDataCompressor", b"vtkLZ4DataCompressor")
+    open(path, "wb").write(txt)
+    with pytest.raises(ValueError):
+        io.read_vti(path)
+
+
 def test_pvd_collection(tmp_path):
     """VTKWriter appends one file per save with timestep = sim_time."""
     Ng = (6, 6)
