@@ -52,12 +52,14 @@ typedef struct {
     int D, Ng[3], per[3];
     long st[3];
     size_t ns;
+    int exitbc; /* WaterLily exitBC: BC! leaves the exit face (the upper x face of u_x) alone */
 } grid_t;
 
 typedef struct { int lo[3], hi[3]; } rng_t;
 
 static void grid_init(grid_t *g, int D, const int *Ng, int perdir) {
     g->D = D;
+    g->exitbc = 0;
     for (int d = 0; d < 3; ++d) {
         g->Ng[d] = d < D ? Ng[d] : 1;
         g->per[d] = d < D ? ((perdir >> d) & 1) : 0;
@@ -497,7 +499,10 @@ static void bc_vof(const grid_t *g, T *f, T *al, T *nh) {
     }
 }
 /* WaterLily BC!(a,A,false,perdir) for a staggered vector field */
-static void bc_vec(const grid_t *g, T *a, const T *A) {
+/* WaterLily BC!(a, A, saveexit, perdir): periodic copies; normal component
+ * Dirichlet A on slices 1, 2 and N (the exit face N of u_x skipped when
+ * saveexit); tangential components Neumann */
+static void bc_vec_e(const grid_t *g, T *a, const T *A, int saveexit) {
     for (int ii = 1; ii <= g->D; ++ii) {
         T *ai = VC(a, ii);
         for (int jj = 1; jj <= g->D; ++jj) {
@@ -508,7 +513,7 @@ static void bc_vec(const grid_t *g, T *a, const T *A) {
             } else if (ii == jj) {
                 FOR_R(R1) ai[IX(i, j, k)] = A[ii - 1];
                 FOR_R(R2) ai[IX(i, j, k)] = A[ii - 1];
-                FOR_R(R3) ai[IX(i, j, k)] = A[ii - 1];
+                if (!saveexit || ii > 1) FOR_R(R3) ai[IX(i, j, k)] = A[ii - 1];
             } else {
                 FOR_R(R1) { size_t I = IX(i, j, k); ai[I] = ai[I + ST(jj)]; }
                 FOR_R(R3) { size_t I = IX(i, j, k); ai[I] = ai[I - ST(jj)]; }
@@ -516,6 +521,8 @@ static void bc_vec(const grid_t *g, T *a, const T *A) {
         }
     }
 }
+/* BC!(a, A, exitBC, perdir) as the step calls it (flow.jl:69, :79, :82, :197, :207) */
+static void bc_vec(const grid_t *g, T *a, const T *A) { bc_vec_e(g, a, A, g->exitbc); }
 /* WaterLily perBC! */
 static void per_bc(const grid_t *g, T *a) {
     for (int jj = 1; jj <= g->D; ++jj) {
@@ -770,7 +777,7 @@ static void mg_restrictL(lev_t *c, const lev_t *fl) {
         }
     }
     T z3[3] = {0, 0, 0};
-    bc_vec(g, c->L, z3);
+    bc_vec_e(g, c->L, z3, 0);
 }
 /* prolongate!(a,b): a[I] = b[down(I)],  down(I) = (I+2)÷2 */
 static void mg_prolongate(lev_t *fl, const lev_t *c) {
@@ -1421,7 +1428,7 @@ static void update_L(oc_ctx *c, const T *f) {
         T *Ld = VC(c->mu0, d);
         PFOR_R(R) { size_t I = IX(i, j, k); Ld[I] /= lin((f[I] + f[I - ST(d)]) / 2, c->lr); }
     }
-    bc_vec(g, c->mu0, c->zero3);
+    bc_vec_e(g, c->mu0, c->zero3, 0); /* BC!(μ₀, zeros, false, perdir), flow.jl:258 */
 }
 /* flow.jl:262-281 MPCFL (+ WaterLily flux_out, whose `0.` literal is Float64) */
 static T mpcfl(oc_ctx *c) {
@@ -1469,8 +1476,58 @@ static void proj_apply(oc_ctx *c, T dt) {
     for (size_t q = 0; q < g->ns; ++q) c->p[q] = c->p[q] * idt;
 }
 int oc_project_fcg(oc_ctx *c, double dt_w, double rtol, int max_it, int smooth_it, int *iters, double *rel);
+/* flow.jl:300-326 psolver!(p; tol, itmx) on the single-level Poisson (the
+ * finest MultiLevelPoisson level: same L, D, iD, x, ϵ, r, z): Jacobi-
+ * preconditioned CG without early exits, stopping on the absolute Σr² */
+static int psolver(oc_ctx *c, T tol, int itmx, double *r2o) {
+    lev_t *lv = &c->lev[0];
+    grid_t *g = &lv->g;
+    rng_t R = r_in(g), A = r_all(g);
+    T *x = lv->x, *r = lv->r, *e = lv->eps, *z = lv->z;
+    per_bc(g, x);
+    mg_residual(lv);
+    T r2 = (T)r_dot(g, R, r, r);
+    int n = 0;
+    PFOR_R(R) { size_t I = IX(i, j, k); z[I] = e[I] = r[I] * lv->iD[I]; }
+    T rho = (T)r_dot(g, A, r, z);
+    while ((r2 > tol || (r2 > tol / 4 && n == 0)) && n < itmx) {
+        per_bc(g, e);
+        PFOR_R(R) { size_t I = IX(i, j, k); z[I] = mg_mult(g, I, lv->L, lv->Dg, e); }
+        T alpha = rho / (T)r_dot(g, R, z, e);
+        PFOR_R(R) {
+            size_t I = IX(i, j, k);
+            x[I] += alpha * e[I];
+            r[I] -= alpha * z[I];
+        }
+        PFOR_R(R) { size_t I = IX(i, j, k); z[I] = r[I] * lv->iD[I]; }
+        T rho2 = (T)r_dot(g, A, r, z);
+        T beta = rho2 / rho;
+        PFOR_R(R) { size_t I = IX(i, j, k); e[I] = beta * e[I] + z[I]; }
+        rho = rho2;
+        r2 = (T)r_dot(g, R, r, r);
+        n++;
+    }
+    per_bc(g, x);
+    if (r2o) *r2o = (double)r2;
+    return n;
+}
 /* flow.jl:328-335 myproject! */
 static int myproject(oc_ctx *c, T dt, int *iters, double *r2) {
+    if (c->poisson_mode == 2) {
+        /* inproject!(a, b::Poisson, dt) (flow.jl:337-341) */
+        grid_t *g = &c->g;
+        rng_t R = r_in(g);
+        lev_t *l0 = &c->lev[0];
+        memset(c->sig, 0, g->ns * sizeof(T));
+        memset(l0->eps, 0, g->ns * sizeof(T));
+        memset(l0->r, 0, g->ns * sizeof(T));
+        PFOR_R(R) { size_t I = IX(i, j, k); c->sig[I] = divu(g, c->u, I); }
+        for (size_t q = 0; q < g->ns; ++q) c->p[q] = c->p[q] * dt;
+        int n = psolver(c, (T)(50 * EPS_T), 2000, r2);
+        if (iters) *iters = n;
+        proj_apply(c, dt);
+        return OC_OK;
+    }
     if (c->poisson_mode == 1) {
         double rel;
         int e = oc_project_fcg(c, (double)dt, c->poisson_rtol, 200, c->smooth_it, iters, &rel);
@@ -1560,6 +1617,7 @@ oc_ctx *oc_create(const oc_params *p) {
     int Ng[3];
     for (int d = 0; d < 3; ++d) Ng[d] = d < p->D ? p->N[d] + 2 : 1;
     grid_init(&c->g, p->D, Ng, p->perdir);
+    c->g.exitbc = p->exit_bc && !(p->perdir & 1);
     grid_t *g = &c->g;
     c->D = p->D;
     for (int d = 0; d < 3; ++d) {
@@ -1604,7 +1662,7 @@ oc_ctx *oc_create(const oc_params *p) {
     c->p = alloc_fill(n, 0);
     c->sig = alloc_fill(n, 0);
     c->mu0 = alloc_fill(nv, 1);
-    bc_vec(g, c->mu0, c->zero3);
+    bc_vec_e(g, c->mu0, c->zero3, 0); /* BC!(μ₀, zeros, false, perdir), flow.jl:258 */
     /* cVOF.jl:45-89 */
     c->f = alloc_fill(n, 1);
     c->al = alloc_fill(n, 0);
@@ -2098,6 +2156,12 @@ int oc_bc_scalar(int D, const int *Ng, real_t *f, int perdir, int d) {
     grid_t g;
     grid_init(&g, D, Ng, perdir);
     bc_f(&g, f, d);
+    return OC_OK;
+}
+int oc_bc_vel(int D, const int *Ng, real_t *a, const real_t *A, int perdir, int saveexit) {
+    grid_t G, *g = &G;
+    grid_init(g, D, Ng, perdir);
+    bc_vec_e(g, a, A, saveexit);
     return OC_OK;
 }
 int oc_bc_facevec(int D, const int *Ng, real_t *v, int perdir) {

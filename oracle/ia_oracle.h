@@ -42,9 +42,11 @@ typedef struct oc_params {
     double dt0;         /* initial Δt of the Simulation (WaterLily default 0.25) */
     int mg_itmx;        /* solver! itmx (reference: 200) */
     double mg_tol;      /* solver! tol on Σr² (reference: 1e-4) */
-    int poisson_mode;   /* 0: WaterLily solver! (reference); 1: FP64 flexible CG + V-cycle to poisson_rtol */
+    int poisson_mode;   /* 0: WaterLily solver! (reference); 1: FP64 flexible CG + V-cycle to poisson_rtol;
+                           2: psolver! on the single-level Poisson (flow.jl:300-341) */
     double poisson_rtol;
     int smooth_it;      /* fine-level pcg! iterations per preconditioning V-cycle (mode 1) */
+    int exit_bc;        /* WaterLily exitBC: BC! keeps the exit (upper x) face of u_x */
 } oc_params;
 
 /* field ids */
@@ -135,6 +137,9 @@ int oc_redistance(int D, const int *Ng, real_t *phi, real_t *phi0, const real_t 
                   int perdir);
 void oc_set_threads(int n);
 const char *oc_last_error(oc_ctx *c);
+
+/* WaterLily BC!(a, A, saveexit, perdir) on a staggered vector field (the step's exitBC path) */
+int oc_bc_vel(int D, const int *Ng, real_t *a, const real_t *A, int perdir, int saveexit);
 
 #ifdef __cplusplus
 }

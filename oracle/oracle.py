@@ -30,7 +30,7 @@ class OcParams(C.Structure):
                 ("nu", C.c_double), ("lambda_rho", C.c_double), ("lambda_mu", C.c_double),
                 ("normal_scheme", C.c_int), ("limiter", C.c_int), ("dt0", C.c_double),
                 ("mg_itmx", C.c_int), ("mg_tol", C.c_double), ("poisson_mode", C.c_int),
-                ("poisson_rtol", C.c_double), ("smooth_it", C.c_int)]
+                ("poisson_rtol", C.c_double), ("smooth_it", C.c_int), ("exit_bc", C.c_int)]
 
 
 _libs = {}
@@ -94,6 +94,7 @@ def lib(T):
         "oc_getmu": (R, [C.c_int, ip, vp, C.c_int, C.c_int, ip, R, R, R]),
         "oc_bc_scalar": (C.c_int, [C.c_int, ip, vp, C.c_int, C.c_int]),
         "oc_bc_facevec": (C.c_int, [C.c_int, ip, vp, C.c_int]),
+        "oc_bc_vel": (C.c_int, [C.c_int, ip, vp, vp, C.c_int, C.c_int]),
         "oc_bc_vof": (C.c_int, [C.c_int, ip, vp, vp, vp, C.c_int]),
         "oc_f2face": (C.c_int, [C.c_int, ip, vp, vp, C.c_int]),
         "oc_vof_flux_face": (C.c_int, [C.c_int, ip, vp, vp, vp, vp, R, C.c_int, ip, vp, R]),
@@ -206,7 +207,8 @@ class OracleSim:
 
     def __init__(self, dims, u_BC, L=1.0, *, T=np.float32, λμ=1e-2, λρ=1e-3, ν=0.0, g=None,
                  perdir=(), Δt=0.25, InterfaceSDF=None, u0=None, normal="WH", limiter="Koren",
-                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=2, η=None):
+                 mg_tol=1e-4, mg_itmx=200, U=None, poisson="reference", poisson_rtol=1e-6, smooth_it=2, η=None,
+                 exitBC=False):
         self.T = np.dtype(T)
         self.lib = lib(self.T)
         self.dims = tuple(int(n) for n in dims)
@@ -231,9 +233,10 @@ class OracleSim:
         p.dt0 = float(Δt)
         p.mg_itmx = int(mg_itmx)
         p.mg_tol = float(mg_tol)
-        p.poisson_mode = {"reference": 0, "refined": 1}[poisson]
+        p.poisson_mode = {"reference": 0, "refined": 1, "pcg": 2}[poisson]
         p.poisson_rtol = float(poisson_rtol)
         p.smooth_it = int(smooth_it)
+        p.exit_bc = 1 if exitBC else 0
         self.ctx = self.lib.oc_create(C.byref(p))
         if not self.ctx:
             raise ValueError("oracle: bad parameters (MultiLevelPoisson needs >2 levels)")

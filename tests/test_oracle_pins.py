@@ -241,6 +241,37 @@ def test_poisson_converges_to_direct_solve(perdir):
     assert (dv ** 2).sum() <= 1e-15
 
 
+@pytest.mark.parametrize("perdir", [(), (1, 2)])
+def test_psolver_converges_to_direct_solve(perdir):
+    """psolver! (flow.jl:300-326) through inproject!(::Poisson) (flow.jl:337-341):
+    plain Jacobi-PCG until Σr² <= 50eps(T) equals the sparse direct solve of
+    the same operator; the projected field is divergence free."""
+    import ctypes as C
+    rng = np.random.default_rng(7)
+    N = 16
+    sdf = lambda x: np.sqrt((x[0] - 7.3) ** 2 + (x[1] - 9.1) ** 2) - 4.2
+    sim = O.OracleSim((N, N), (0.0, 0.0), T=np.float64, InterfaceSDF=sdf, perdir=perdir, λρ=1e-3,
+                      poisson="pcg")
+    u = np.zeros(sim.Ng + (2,), order="F")
+    u[1:-1, 1:-1, :] = rng.standard_normal((N, N, 2))
+    sim.lib.oc_set_initial_velocity(sim.ctx, O._ptr(u))
+    sim.lib.oc_update_L(sim.ctx, 0)
+    sim.lib.oc_mg_update(sim.ctx)
+    z = _div(sim.get(O.U), N)
+    itc, r2c = C.c_int(), C.c_double()
+    sim.lib.oc_project(sim.ctx, C.c_double(1.0), C.byref(itc), C.byref(r2c))
+    assert r2c.value <= 50 * np.finfo(np.float64).eps
+    assert 10 < itc.value < 2000
+    x = sim.get(O.P)[1:-1, 1:-1].ravel(order="F")
+    A = _poisson_matrix(sim)
+    zz = z.ravel(order="F") - z.mean()
+    sol = spla.lsqr(A, zz, atol=1e-15, btol=1e-15, iter_lim=50000)[0]
+    sol = sol - sol.mean() + x.mean()
+    assert np.linalg.norm(x - sol) / np.linalg.norm(sol) < 1e-9
+    dv = _div(sim.get(O.U), N)
+    assert (dv ** 2).sum() <= 1e-13
+
+
 def _div(u, N):
     dv = np.zeros((N, N))
     for i in range(2):

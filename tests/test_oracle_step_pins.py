@@ -17,6 +17,10 @@ what the mathematics fixes (CPU only, FP64 oracle):
   p_j - p_{j-1} = ρ_face g_y on every vertical face.
 - updateU! in a periodic box (flow.jl:244-252, :74-77): free fall, u = gΔt,
   and the interface translated by the predictor's gΔt²/2.
+- WaterLily BC!(a, A, exitBC, perdir) (Appendix of the survey; flow.jl:69,
+  :79, :82, :197, :207): with exitBC the exit face of u_x (the upper x face)
+  keeps its value, and the tangential Neumann images at that face copy it;
+  everything else is as without.
 - reportFillError (advection.jl:145-189): a strongly divergent velocity at the
   interface (|∇·u⁰| + |∇·u| > 10 where f leaves [0, 1]) returns the explosion
   status, exactly 10 only the fill warning.
@@ -254,3 +258,30 @@ def test_divergent_flow_at_the_interface_explodes():
     assert _squeeze(3.0, 0.4 / 3.0) == 2
     assert _squeeze(2.5, 0.4 / 2.5) == 3
     assert _squeeze(3.0, 0.04 / 3.0) == 0
+
+
+# ----------------------------------------------------------------------------- BC! with exitBC
+@pytest.mark.parametrize("D", [2, 3])
+def test_bc_exit_face_is_left_alone(D):
+    rng = np.random.default_rng(12)
+    Ng = (10, 8, 6)[:D]
+    a = np.asfortranarray(rng.standard_normal(Ng + (D,)))
+    A = np.array([0.7, -0.2, 0.3])
+    L = O.lib(np.float64)
+    plain, kept = a.copy(order="F"), a.copy(order="F")
+    L.oc_bc_vel(D, O._ints(Ng), O._ptr(plain), O._ptr(A), 0, 0)
+    L.oc_bc_vel(D, O._ints(Ng), O._ptr(kept), O._ptr(A), 0, 1)
+    nx = Ng[0]
+    # without exitBC: every normal face on slices 1, 2 and N is A
+    assert np.all(plain[0, ..., 0] == A[0]) and np.all(plain[1, ..., 0] == A[0]) and np.all(plain[-1, ..., 0] == A[0])
+    # with it: the exit slice of u_x keeps its interior-row values, and its
+    # tangential ghosts copy their interior neighbours
+    ex = a[-1, ..., 0].copy()
+    inner = tuple(slice(1, n - 1) for n in Ng[1:])
+    assert np.array_equal(kept[(-1,) + inner + (0,)], ex[inner])
+    assert np.array_equal(kept[-1, 0, ..., 0], kept[-1, 1, ..., 0])
+    assert np.array_equal(kept[-1, -1, ..., 0], kept[-1, -2, ..., 0])
+    # and everything else is as without exitBC
+    mask = np.ones(kept.shape, bool)
+    mask[-1, ..., 0] = False
+    assert np.array_equal(kept[mask], plain[mask])
