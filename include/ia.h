@@ -54,11 +54,17 @@ enum ia_limiter {
  *            Chebyshev smoothing steps on the finest level) per iteration;
  *            same stopping test, FP64 residual and pressure accumulation.  No
  *            reference counterpart.
+ * PCG:       inproject!(a, b::Poisson, dt) (flow.jl:337-341): psolver! (flow.jl:300-326),
+ *            Jacobi-preconditioned CG on the single (finest) level until the
+ *            absolute Σr² <= 50eps(T) or 2000 iterations, as the reference runs it
+ *            when the simulation's Poisson is a single-level `Poisson`.
  * LEARNED:   REFINED, whose warm start is first corrected by the learned network
  *            (ia_learned_*; no reference counterpart).  The correction is kept
  *            only when it lowers the FP64 residual.  3D, FP32, N[0] % 16 == 0,
  *            N[1] and N[2] even. */
-enum ia_poisson_mode { IA_POISSON_REFERENCE = 0, IA_POISSON_REFINED = 1, IA_POISSON_LEARNED = 2, IA_POISSON_PLAIN = 3 };
+enum ia_poisson_mode {
+    IA_POISSON_REFERENCE = 0, IA_POISSON_REFINED = 1, IA_POISSON_LEARNED = 2, IA_POISSON_PLAIN = 3, IA_POISSON_PCG = 4
+};
 
 /* Status codes (advection.jl:145-189 reportFillError and the glue's errors). */
 enum ia_status {
@@ -107,6 +113,9 @@ typedef struct ia_params {
     double poisson_rtol;   /* REFINED/LEARNED: relative FP64 residual (e.g. 1e-6) */
     int poisson_max_outer; /* REFINED/LEARNED: outer iteration cap */
     int device;            /* HIP device ordinal */
+    int exit_bc;           /* WaterLily exitBC: the step's BC! leaves u_x on the exit face (the
+                              upper x boundary) alone (flow.jl:69, :79, :82, :197, :207); needs
+                              no gravity along x (g[0] = 0) */
 } ia_params;
 
 typedef struct ia_solve_stats {
@@ -133,11 +142,22 @@ typedef struct ia_step_stats {
 int ia_create(const ia_params *p, ia_ctx **out);
 void ia_destroy(ia_ctx *c);
 
-/* Copy a whole field in/out (nbytes must equal the field's size). */
+/* Copy a whole field in/out (nbytes must equal the field's size); the other
+ * buffer may be host memory or device memory (unified addressing). */
 int ia_set_field(ia_ctx *c, int field, const void *host, size_t nbytes);
 int ia_get_field(ia_ctx *c, int field, void *host, size_t nbytes);
 /* Zero-copy access to the current device buffer of a field. */
 int ia_field_device_ptr(ia_ctx *c, int field, void **dptr, size_t *nbytes);
+/* Zero-copy adoption of a caller's device buffer (on the context's device,
+ * nbytes = the field's size) as the storage of cVOF.f, cVOF.f⁰, Flow.u,
+ * Flow.u⁰ or Flow.p from now on: a host whose arrays already live on the GPU
+ * (the reference's ROCArrays, InterfaceAdvection.jl:59-60 `mem`) hands them
+ * over instead of copying.  The buffer's contents become the field's value;
+ * every later call returns with the field's current value in it (the sweeps'
+ * f/f⁰ rotation ends with device copies back into attached buffers).  The
+ * library never frees it: the caller keeps it alive until ia_destroy.  No
+ * reference counterpart (the reference's arrays are its own). */
+int ia_attach_field(ia_ctx *c, int field, void *dptr, size_t nbytes);
 /* Sizes of MultiLevelPoisson level l (ghost-inclusive); returns the level count. */
 int ia_mg_levels(ia_ctx *c, int l, int *Ng);
 

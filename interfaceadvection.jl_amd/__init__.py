@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libia.so")
 IA_F32, IA_F64 = 0, 1
 FIELD = dict(f=0, f0=1, alpha=2, nhat=3, cbar=5, rhou=6, rhouf=7, u=9, u0=10, force=11, p=12, sigma=13, mu0=14)
 _VECTOR = {"nhat", "rhou", "rhouf", "u", "u0", "force", "mu0"}
-POISSON = {"reference": 0, "refined": 1, "learned": 2, "plain": 3}
+POISSON = {"reference": 0, "refined": 1, "learned": 2, "plain": 3, "pcg": 4}
 PROFILE = {"vof": 0, "momentum": 1, "visc_update": 2, "pcg_Ae": 3, "pcg_update": 4, "increment": 5,
            "load_inc": 6, "prolong_inc": 7, "fcg_Ap": 8, "fcg_update": 9, "pcg_last": 10, "cheb2_dots": 10, "redist": 11,
            "cg1_l1": 12, "cg1_l2": 13, "cg1_l3": 14}
@@ -35,7 +35,7 @@ class IAParams(C.Structure):
                 ("lambda_rho", C.c_double), ("lambda_mu", C.c_double), ("eta", C.c_double),
                 ("normal_scheme", C.c_int), ("limiter", C.c_int), ("poisson_mode", C.c_int),
                 ("poisson_tol", C.c_double), ("poisson_itmx", C.c_int), ("poisson_rtol", C.c_double),
-                ("poisson_max_outer", C.c_int), ("device", C.c_int)]
+                ("poisson_max_outer", C.c_int), ("device", C.c_int), ("exit_bc", C.c_int)]
 
 
 class IASolveStats(C.Structure):
@@ -73,6 +73,7 @@ def lib():
         "ia_set_field": (C.c_int, [vp, C.c_int, vp, sz]),
         "ia_get_field": (C.c_int, [vp, C.c_int, vp, sz]),
         "ia_field_device_ptr": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.POINTER(sz)]),
+        "ia_attach_field": (C.c_int, [vp, C.c_int, vp, sz]),
         "ia_mg_levels": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
         "ia_apply_vof": (C.c_int, [vp, vp, sz]),
         "ia_set_initial_velocity": (C.c_int, [vp, vp, sz]),
@@ -269,8 +270,6 @@ class TwoPhaseSimulation:
         self.T = np.dtype(T)
         if self.T not in (np.float32, np.float64):
             raise ValueError("T must be Float32 or Float64")
-        if exitBC:
-            raise ValueError("exitBC is not supported by this backend")
         if body is not None:
             raise ValueError("immersed bodies are not supported by MPFMomStep! (InterfaceAdvection.jl:97)")
         if η is not None and not (float(η) > 0):
@@ -306,6 +305,9 @@ class TwoPhaseSimulation:
         p.poisson_rtol = float(poisson_rtol)
         p.poisson_max_outer = int(poisson_max_outer)
         p.device = int(device)
+        p.exit_bc = 1 if exitBC else 0
+        if exitBC and gv is not None and gv[0] != 0:
+            raise ValueError("exitBC with gravity along x is not supported (the exit face's uStar would be g_x)")
         self.params = p
         self._lib = lib()
         ctx = C.c_void_p()

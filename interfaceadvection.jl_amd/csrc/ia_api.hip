@@ -96,9 +96,6 @@ struct ia_ctx {
     double *pp = nullptr;      // partial-sum buffers (PP_N of pp_stride doubles)
     long pp_stride = 0;
     std::vector<double> hpart; // host copy of a partial-sum buffer
-    hipGraphExec_t fcg_graph[4] = {};   // one flexible-CG iteration: [first][p-buffer parity]
-    bool capturing = false;
-    int use_graphs = 0;    // IA_USE_GRAPHS=1: replay each FCG iteration as a HIP graph
     double *dsc = nullptr;   // device scalars
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *pa64 = nullptr, *pb64 = nullptr, *q64 = nullptr;
     int fine_cheb = 1;   // the finest level's smoother after each preconditioning V-cycle: two Chebyshev steps
@@ -128,6 +125,10 @@ struct ia_ctx {
     const int *halt = nullptr;   // the flexible CG's stop flag while a pipelined solve runs
     FcgState *pin = nullptr;     // pinned host copies of the solver state (two slots)
     hipEvent_t it_ev[2] = {nullptr, nullptr};
+    // caller-owned device buffers adopted by ia_attach_field (nullptr: the
+    // library's own); att_fi/att_f0i: the f-pool slots holding attached f, f⁰
+    void *att_u = nullptr, *att_u0 = nullptr, *att_p = nullptr;
+    int att_fi = -1, att_f0i = -1;
     int tail_level = 99; // first level handled by the one-workgroup tail kernel
     size_t tail_lds = 0; // its dynamic LDS bytes
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -165,7 +166,7 @@ int fail(ia_ctx *c, int code, const char *fmt, ...) {
 // HIP events around the launches of one kernel class (ia_profile_enable)
 void prof_begin(ia_ctx *c, int k) {
     auto &p = c->prof[k];
-    if (!p.on || c->capturing) return;
+    if (!p.on) return;
     while ((int)p.ev.size() < 2 * (p.n + 1)) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) { p.on = false; return; }
@@ -177,7 +178,7 @@ void prof_begin(ia_ctx *c, int k) {
 // run of back-to-back launches costs two events instead of two per launch)
 void prof_end(ia_ctx *c, int k, int nlaunch = 1) {
     auto &p = c->prof[k];
-    if (!p.on || c->capturing) return;
+    if (!p.on) return;
     (void)hipEventRecord(p.ev[2 * p.n + 1], c->s);
     p.n++;
     p.launches += nlaunch;
@@ -185,6 +186,7 @@ void prof_end(ia_ctx *c, int k, int nlaunch = 1) {
 
 void grid_init(Grid &g, int D, const int *Ng, int perdir) {
     g.D = D;
+    g.exitx = 0;
     for (int a = 0; a < 3; ++a) {
         g.Ng[a] = a < D ? Ng[a] : 1;
         g.per[a] = a < D ? ((perdir >> a) & 1) : 0;
@@ -204,9 +206,12 @@ Tiling tiling(const Grid &g) {
     t.nblk = (long)t.nt[0] * t.nt[1] * t.nt[2];
     return t;
 }
-CTiling ctiling(const Grid &g) {
+// narrow: a coarse level may take 32-wide rows (only the one-cell-per-thread
+// kernels run there)
+CTiling ctiling(const Grid &g, bool narrow = false) {
     CTiling t;
     int n0 = g.Ng[0] - 2, n1 = g.D > 1 ? g.Ng[1] - 2 : 1, n2 = g.D > 2 ? g.Ng[2] - 2 : 1;
+    t.nx = TX;
     t.nt[0] = (n0 + TX - 1) / TX;
     t.nt[1] = (n1 + TY - 1) / TY;
     long base = (long)t.nt[0] * t.nt[1];
@@ -217,7 +222,13 @@ CTiling ctiling(const Grid &g) {
     if (g.D > 2) zb = (int)std::max(1L, std::min(32L, base * n2 / 1024));
     t.zb = zb;
     t.nt[2] = g.D > 2 ? (n2 + zb - 1) / zb : 1;
-    t.nblk = base * t.nt[2];
+    if (narrow && g.D == 3 && zb == 1 && n0 <= 32) {
+        t.nx = 32;
+        t.nt[0] = 1;
+        t.nt[1] = (n1 + NT / 32 - 1) / (NT / 32);
+        t.nt[2] = n2;
+    }
+    t.nblk = (long)t.nt[0] * t.nt[1] * t.nt[2];
     return t;
 }
 long interior(const Grid &g) {
@@ -263,6 +274,63 @@ int free_fbuf(ia_ctx *c, int a, int b, int d2) {
         if (q != a && q != b && q != d2) return q;
     return -1;
 }
+bool is_attached(const ia_ctx *c, const void *b) {
+    if (!b) return false;
+    if (b == c->att_u || b == c->att_u0 || b == c->att_p) return true;
+    if (c->att_fi >= 0 && b == c->fb[c->att_fi]) return true;
+    if (c->att_f0i >= 0 && b == c->fb[c->att_f0i]) return true;
+    return false;
+}
+// the sweeps rotate f and f⁰ through the four pool buffers; when the caller
+// attached its own f or f⁰ buffers, move the current values back into them
+// (device copies on the context's stream) so every call returns with
+// cVOF.f and cVOF.f⁰ in the caller's arrays
+int restore_attached(ia_ctx *c) {
+    int a = c->att_fi, b = c->att_f0i, x = c->fi, y = c->f0i;
+    if ((a < 0 || x == a) && (b < 0 || y == b)) return IA_OK;
+    size_t bytes = (size_t)c->g.ns * c->es;
+    auto cp = [&](int dst, int src) -> hipError_t {
+        return hipMemcpyAsync(c->fb[dst], c->fb[src], bytes, hipMemcpyDeviceToDevice, c->s);
+    };
+    hipError_t e = hipSuccess;
+    if (a >= 0 && b >= 0) {
+        if (x == b && y == a) {  // exchanged: through a third buffer
+            int t = free_fbuf(c, a, b, -1);
+            if (e == hipSuccess) e = cp(t, a);
+            if (e == hipSuccess) e = cp(a, b);
+            if (e == hipSuccess) e = cp(b, t);
+        } else if (x == b) {     // f sits in f⁰'s slot, f⁰ elsewhere
+            if (e == hipSuccess) e = cp(a, b);
+            if (e == hipSuccess) e = cp(b, y);
+        } else if (y == a) {     // f⁰ sits in f's slot, f elsewhere
+            if (e == hipSuccess) e = cp(b, a);
+            if (e == hipSuccess) e = cp(a, x);
+        } else {
+            if (x != a && e == hipSuccess) e = cp(a, x);
+            if (y != b && e == hipSuccess) e = cp(b, y);
+        }
+        c->fi = a;
+        c->f0i = b;
+    } else if (a >= 0) {
+        if (y == a) {            // f⁰ moves to a free slot first
+            int t = free_fbuf(c, a, x, -1);
+            if (e == hipSuccess) e = cp(t, a);
+            c->f0i = t;
+        }
+        if (e == hipSuccess) e = cp(a, x);
+        c->fi = a;
+    } else {
+        if (x == b) {
+            int t = free_fbuf(c, b, y, -1);
+            if (e == hipSuccess) e = cp(t, b);
+            c->fi = t;
+        }
+        if (e == hipSuccess) e = cp(b, y);
+        c->f0i = b;
+    }
+    if (e != hipSuccess) return fail(c, IA_ERR_HIP, "restoring attached f/f0: %s", hipGetErrorString(e));
+    return IA_OK;
+}
 
 // partial-sum buffers
 enum { PP_A = 0, PP_B, PP_C, PP_MR, PP_ML2, PP_FI, PP_FZ, PP_FR, PP_FRR, PP_FD, PP_FAP, PP_FU, PP_N };
@@ -270,14 +338,19 @@ enum { PP_A = 0, PP_B, PP_C, PP_MR, PP_ML2, PP_FI, PP_FZ, PP_FR, PP_FRR, PP_FD, 
 template <class T> T *P(void *v) { return (T *)v; }
 template <class T> const T *P(const void *v) { return (const T *)v; }
 
-// WaterLily BC!(a, A) on a staggered vector field whose interior is current
+// WaterLily BC!(a, A, saveexit) on a staggered vector field whose interior is
+// current (kind 1: saveexit false; kind 2: the velocity's exitBC, the exit
+// face keeping its own value)
 template <class T>
-__global__ void __launch_bounds__(NT) k_bcvel(Grid g, Tiling t, T *a, T d0, T d1, T d2) {
+__global__ void __launch_bounds__(NT) k_bcvel(Grid g, Tiling t, T *a, T d0, T d1, T d2, int kind) {
     int p[3];
     if (!thread_cell(g, t, p)) return;
     long I = lin(g, p);
     T dv[3] = {d0, d1, d2};
-    for (int i = 0; i < g.D; ++i) write_images<T>(g, a + g.ns * i, p, a[g.ns * i + I], 1, i, dv[i]);
+    for (int i = 0; i < g.D; ++i) {
+        T xv = kind == 2 && i == 0 && g.exitx ? a[I + g.st[0]] : T(0);
+        write_images<T>(g, a + g.ns * i, p, a[g.ns * i + I], kind, i, dv[i], xv);
+    }
 }
 
 // metrics integrands, grid-stride over the tiles: every block folds the tiles
@@ -335,8 +408,9 @@ template <class T> struct Impl {
         LAUNCHCHK(c);
         return IA_OK;
     }
-    static int bc_vel(ia_ctx *c, void *a, const T *dv) {
-        hipLaunchKernelGGL(k_bcvel<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(a), dv[0], dv[1], dv[2]);
+    static int bc_vel(ia_ctx *c, void *a, const T *dv, int kind = 2) {
+        hipLaunchKernelGGL(k_bcvel<T>, blocks(c->t), dim3(NT), 0, c->s, c->g, c->t, P<T>(a), dv[0], dv[1], dv[2],
+                           kind);
         LAUNCHCHK(c);
         return IA_OK;
     }
@@ -361,7 +435,7 @@ template <class T> struct Impl {
         bc_vel(c, c->u, ub);
         HIPCHK(c, hipMemcpyAsync(c->u0, c->u, nv * es, hipMemcpyDeviceToDevice, c->s));
         fill(c, c->L, nv, T(1));
-        bc_vel(c, c->L, z3);
+        bc_vel(c, c->L, z3, 1);
         // cVOF: f = 1, f⁰ = 1
         fill(c, c->fb[0], ns, T(1));
         fill(c, c->fb[1], ns, T(1));
@@ -390,7 +464,7 @@ template <class T> struct Impl {
             Level &lv = c->lev[c->nlev];
             grid_init(lv.g, g.D, Na, c->P.perdir);
             lv.t = tiling(lv.g);
-            lv.ct = ctiling(lv.g);
+            lv.ct = ctiling(lv.g, true);
             size_t n = lv.g.ns;
             if (int r = alloc_zero(c, &lv.L, n * g.D * es)) return r;
             void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z, &lv.e2, &lv.z2, &lv.r2, &lv.w, &lv.w2};
@@ -655,6 +729,58 @@ template <class T> struct Impl {
         }
         return IA_OK;
     }
+    // inproject!(a, b::Poisson, dt) + myproject! + BC!(u) (flow.jl:300-341, :82):
+    // psolver!(b; tol = 50eps(T), itmx = 2000) on the single-level Poisson
+    static int project_pcg(ia_ctx *c, void *u, double dtw_d, ia_solve_stats *ss) {
+        Grid &g = c->g;
+        Level &l0 = c->lev[0];
+        T dtw = (T)dtw_d;
+        long ncell = interior(g), nb = l0.ct.nblk;
+        // fill!(b.z, 0); fill!(b.ϵ, 0); fill!(b.r, 0); z = div u; x *= dt
+        fill(c, c->sig, g.ns, T(0));
+        fill(c, l0.e, g.ns, T(0));
+        fill(c, l0.r, g.ns, T(0));
+        hipLaunchKernelGGL(k_div<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->sig));
+        hipLaunchKernelGGL(k_scale<T>, blocks1(g.ns), dim3(256), 0, c->s, P<T>(c->p), g.ns, dtw);
+        // residual!(p); r₂ = L₂(p)   (x's periodic images are read through the wrapped offsets)
+        hipLaunchKernelGGL(k_mg_resid<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR));
+        hipLaunchKernelGGL(k_mg_center_l2<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_MR), ncell, 1,
+                           pp(c, PP_ML2));
+        // z = ϵ = r·iD, ρ = r·z
+        hipLaunchKernelGGL(k_pcg_init<T>, cb(l0), dim3(NT), 0, c->s, view(l0), pp(c, PP_A));
+        double *rho = c->dsc + 20;  // two slots: 20, 21
+        hipLaunchKernelGGL(k_ps_rho<T>, dim3(1), dim3(NT), 0, c->s, pp(c, PP_A), nb, rho);
+        LAUNCHCHK(c);
+        double r2;
+        if (int e = host_total(c, PP_ML2, nb, 1, &r2)) return e;
+        r2 = (double)(T)r2;
+        const double tol = (double)(T)(50 * epsT());
+        const int itmx = 2000;
+        int n = 0;
+        while ((r2 > tol || (r2 > tol / 4 && n == 0)) && n < itmx) {
+            LevV<T> v = view(l0);
+            hipLaunchKernelGGL(k_ps_Ae<T>, cb(l0), dim3(NT), 0, c->s, v, pp(c, PP_B));
+            hipLaunchKernelGGL(k_ps_update<T>, cb(l0), dim3(NT), 0, c->s, v, rho + (n & 1), pp(c, PP_B), pp(c, PP_C));
+            hipLaunchKernelGGL(k_ps_eps<T>, cb(l0), dim3(NT), 0, c->s, v, rho + (n & 1), rho + ((n + 1) & 1),
+                               pp(c, PP_C));
+            LAUNCHCHK(c);
+            double tot[2];
+            if (int e = host_total(c, PP_C, nb, 2, tot)) return e;
+            r2 = (double)(T)tot[1];
+            n++;
+        }
+        hipLaunchKernelGGL(k_mg_perbc<T>, cb(l0), dim3(NT), 0, c->s, view(l0), P<T>(l0.x));
+        hipLaunchKernelGGL(k_project<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(u), P<T>(c->L), P<T>(c->p),
+                           (T)c->P.u_bc[0], (T)c->P.u_bc[1], (T)c->P.u_bc[2]);
+        hipLaunchKernelGGL(k_scale<T>, blocks1(g.ns), dim3(256), 0, c->s, P<T>(c->p), g.ns, T(1) / dtw);
+        LAUNCHCHK(c);
+        if (ss) {
+            ss->iters = n;
+            ss->r2 = r2;
+            ss->rel = 0;
+        }
+        return IA_OK;
+    }
     // FP64 flexible CG preconditioned by the FP32 V-cycle (REFINED / LEARNED)
     static FcgV<T> fview(ia_ctx *c, int par = 0) {
         Level &l0 = c->lev[0];
@@ -716,24 +842,8 @@ template <class T> struct Impl {
         prof_end(c, IA_PROF_FCG_UPDATE);
     }
     static int fcg_iteration(ia_ctx *c, int first, int par, int rbuf) {
-        if (!c->use_graphs || rbuf != (first ? PP_FRR : PP_FU)) {
-            fcg_body(c, first, par, rbuf);
-            LAUNCHCHK(c);
-            return IA_OK;
-        }
-        hipGraphExec_t &ex = c->fcg_graph[2 * first + par];
-        if (!ex) {
-            hipGraph_t gr;
-            c->capturing = true;
-            HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-            fcg_body(c, first, par, rbuf);
-            hipError_t e1 = hipStreamEndCapture(c->s, &gr);
-            c->capturing = false;
-            HIPCHK(c, e1);
-            HIPCHK(c, hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-            HIPCHK(c, hipGraphDestroy(gr));
-        }
-        HIPCHK(c, hipGraphLaunch(ex, c->s));
+        fcg_body(c, first, par, rbuf);
+        LAUNCHCHK(c);
         return IA_OK;
     }
     static int project_refined(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
@@ -871,8 +981,8 @@ template <class T> struct Impl {
         }
         HIPCHK(c, hipMalloc((void **)&n.zbias, CNN_C * sizeof(float)));
         HIPCHK(c, hipMemsetAsync(n.zbias, 0, CNN_C * sizeof(float), c->s));
-        HIPCHK(c, hipMalloc(&n.ff, CNN_L * CNN_T * 64 * sizeof(bf16x4)));
-        HIPCHK(c, hipMalloc(&n.fb, CNN_L * CNN_T * 64 * sizeof(bf16x4)));
+        HIPCHK(c, hipMalloc(&n.ff, CNN_L * CNN_P * 64 * sizeof(bf16x8)));
+        HIPCHK(c, hipMalloc(&n.fb, CNN_L * CNN_P * 64 * sizeof(bf16x8)));
         CnnTiles t = cnn_tiles(c);
         n.wnblk = std::min<long>(1024, t.ntile);
         HIPCHK(c, hipMalloc((void **)&n.wpart, n.wnblk * (CNN_W + CNN_C) * sizeof(float)));
@@ -881,8 +991,8 @@ template <class T> struct Impl {
     static void cnn_frags(ia_ctx *c) {
         auto &n = c->cnn;
         for (int l = 0; l < CNN_L; ++l)
-            hipLaunchKernelGGL(k_cnn_frag, dim3(CNN_T), dim3(64), 0, c->s, n.w + (long)l * CNN_W,
-                               (bf16x4 *)n.ff + l * CNN_T * 64, (bf16x4 *)n.fb + l * CNN_T * 64);
+            hipLaunchKernelGGL(k_cnn_frag, dim3(CNN_P), dim3(64), 0, c->s, n.w + (long)l * CNN_W,
+                               (bf16x8 *)n.ff + l * CNN_P * 64, (bf16x8 *)n.fb + l * CNN_P * 64);
     }
     static CnnBlocks cnn_lds_blocks(ia_ctx *c) {
         CnnBlocks t;
@@ -898,17 +1008,17 @@ template <class T> struct Impl {
         if ((g.Ng[1] - 2) % 4 == 0) {
             CnnBlocks t = cnn_lds_blocks(c);
             hipLaunchKernelGGL(k_cnn_conv_lds, dim3((unsigned)((long)t.tx * t.ty * t.tz)), dim3(256), 0, c->s, t,
-                               (const unsigned short *)X, (unsigned short *)Y, (const bf16x4 *)frag, bias, relu);
+                               (const unsigned short *)X, (unsigned short *)Y, (const bf16x8 *)frag, bias, relu);
         } else {
             CnnTiles t = cnn_tiles(c);
             hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)X,
-                               (unsigned short *)Y, (const bf16x4 *)frag, bias, relu);
+                               (unsigned short *)Y, (const bf16x8 *)frag, bias, relu);
         }
     }
     static void cnn_forward(ia_ctx *c) {
         auto &n = c->cnn;
         for (int l = 0; l < CNN_L; ++l)
-            cnn_conv(c, n.act[l], n.act[l + 1], (const bf16x4 *)n.ff + l * CNN_T * 64,
+            cnn_conv(c, n.act[l], n.act[l + 1], (const bf16x8 *)n.ff + l * CNN_P * 64,
                      n.w + (long)CNN_L * CNN_W + l * CNN_C, l < CNN_L - 1 ? 1 : 0);
     }
     // gradients of every parameter into gw, from the output gradient in grad[0]
@@ -922,7 +1032,7 @@ template <class T> struct Impl {
             hipLaunchKernelGGL(k_cnn_wgrad_sum, dim3((CNN_W + CNN_C + 255) / 256), dim3(256), 0, c->s, n.wpart, n.wnblk,
                                n.gw + (long)l * CNN_W, n.gw + (long)CNN_L * CNN_W + l * CNN_C);
             if (l == 0) break;
-            cnn_conv(c, n.grad[cur], n.grad[1 - cur], (const bf16x4 *)n.fb + l * CNN_T * 64, n.zbias, 0);
+            cnn_conv(c, n.grad[cur], n.grad[1 - cur], (const bf16x8 *)n.fb + l * CNN_P * 64, n.zbias, 0);
             hipLaunchKernelGGL(k_cnn_relu_bwd, dim3((unsigned)((t.ntile * 64 + 255) / 256)), dim3(256), 0, c->s, t,
                                (unsigned short *)n.grad[1 - cur], (const unsigned short *)n.act[l]);
             cur ^= 1;
@@ -1041,7 +1151,9 @@ template <class T> struct Impl {
     }
     static int project(ia_ctx *c, void *u, double dtw, ia_solve_stats *ss) {
         HIPCHK(c, hipEventRecord(c->e0, c->s));
-        int e = c->P.poisson_mode == IA_POISSON_REFERENCE ? project_ref(c, u, dtw, ss) : project_refined(c, u, dtw, ss);
+        int e = c->P.poisson_mode == IA_POISSON_REFERENCE ? project_ref(c, u, dtw, ss)
+                : c->P.poisson_mode == IA_POISSON_PCG     ? project_pcg(c, u, dtw, ss)
+                                                          : project_refined(c, u, dtw, ss);
         if (e) return e;
         HIPCHK(c, hipEventRecord(c->e1, c->s));
         if (ss) {
@@ -1057,9 +1169,10 @@ template <class T> struct Impl {
     // vol (optional): Σf in FP64, min f and max f over the interior of the current f
     static int cfl(ia_ctx *c, double *out, double *vol = nullptr) {
         Grid &g = c->g;
-        hipLaunchKernelGGL(k_cfl<T>, blocks(c->t), dim3(NT), 0, c->s, g, c->t, P<T>(c->u), P<T>(c->fb[c->fi]),
-                           c->part);
-        hipLaunchKernelGGL(k_fin_cfl, dim3(1), dim3(NT), 0, c->s, c->part, c->t.nblk, c->dsc + 8);
+        const CTiling &ct = c->lev[0].ct;
+        hipLaunchKernelGGL(k_cfl<T>, dim3((unsigned)ct.nblk), dim3(NT), 0, c->s, g, ct, P<T>(c->u),
+                           P<T>(c->fb[c->fi]), c->part);
+        hipLaunchKernelGGL(k_fin_cfl, dim3(1), dim3(NT), 0, c->s, c->part, ct.nblk, c->dsc + 8);
         LAUNCHCHK(c);
         double mx[5];
         HIPCHK(c, hipMemcpyAsync(mx, c->dsc + 8, 5 * sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1369,6 +1482,7 @@ template <class T> struct Impl {
         c->u0 = U1;
         c->fi = fnew;
         c->f0i = fn;
+        if (int e = restore_attached(c)) return e;
         double cf, vol[3];
         if (int e = cfl(c, &cf, vol)) return e;
         int st = read_state(c, ss);
@@ -1402,11 +1516,13 @@ template <class T> struct Impl {
             cur = out;
         }
         LAUNCHCHK(c);
-        if (fsrc == c->fi) c->fi = cur;
+        bool adv_f = fsrc == c->fi;
+        if (adv_f) c->fi = cur;
         else c->f0i = cur;
+        if (int e = restore_attached(c)) return e;
         double cf, vol[3];
         int fi_save = c->fi;
-        c->fi = cur;  // the statistics describe the advected field
+        c->fi = adv_f ? c->fi : c->f0i;  // the statistics describe the advected field
         int e = cfl(c, &cf, vol);
         c->fi = fi_save;
         if (e) return e;
@@ -1428,6 +1544,7 @@ template <class T> struct Impl {
         vof_sweep(c, c->fb[c->fi], c->fb[out], c->u0, c->u, d, (T)dtd, 1, 100 * epsT(), nullptr, nullptr);
         LAUNCHCHK(c);
         c->fi = out;
+        if (int e = restore_attached(c)) return e;
         return read_state(c, ss);
     }
     static int visc_stage(ia_ctx *c, int use_f0) {
@@ -1513,8 +1630,11 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     if (!(p->eta >= 0)) return IA_ERR_BADARG;  // η = 0 means η === nothing
     if (p->normal_scheme < 0 || p->normal_scheme > IA_NORMAL_SLIC || p->limiter < 0 || p->limiter > IA_LIMITER_SWEBY)
         return IA_ERR_BADARG;
-    if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PLAIN) return IA_ERR_BADARG;
-    if (p->poisson_mode != IA_POISSON_REFERENCE && p->dtype != IA_F32) return IA_ERR_BADARG;
+    if (p->poisson_mode < 0 || p->poisson_mode > IA_POISSON_PCG) return IA_ERR_BADARG;
+    // exitBC: uStar's exit face is the reference's forcing scratch there, 0 only when g_x = 0
+    if (p->exit_bc && p->has_g && p->g[0] != 0) return IA_ERR_BADARG;
+    if (p->poisson_mode != IA_POISSON_REFERENCE && p->poisson_mode != IA_POISSON_PCG && p->dtype != IA_F32)
+        return IA_ERR_BADARG;
     if (p->poisson_mode == IA_POISSON_LEARNED &&
         !(p->D == 3 && p->N[0] % 16 == 0 && p->N[1] % 2 == 0 && p->N[2] % 2 == 0))
         return IA_ERR_BADARG;  // the network runs on 3D grids of 16-cell rows and even sides
@@ -1526,7 +1646,6 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     if (c->P.poisson_max_outer <= 0) c->P.poisson_max_outer = 100;
     c->dev = p->device;
     c->es = p->dtype == IA_F64 ? 8 : 4;
-    if (const char *gv = getenv("IA_USE_GRAPHS")) c->use_graphs = atoi(gv) != 0;
     if (const char *sv = getenv("IA_FCG_SMOOTH"); sv && *sv) {
         c->smooth_it = std::max(0, std::min(CIT_MAX, atoi(sv)));
         c->fine_cheb = 0;
@@ -1559,6 +1678,7 @@ int ia_create(const ia_params *p, ia_ctx **out) {
     int Ng[3];
     for (int a = 0; a < 3; ++a) Ng[a] = a < p->D ? p->N[a] + 2 : 1;
     grid_init(c->g, p->D, Ng, p->perdir);
+    c->g.exitx = p->exit_bc && !(p->perdir & 1);
     c->t = tiling(c->g);
     if (hipSetDevice(c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
@@ -1588,7 +1708,7 @@ void ia_destroy(ia_ctx *c) {
                     c->L, c->al, c->nh, c->cb, c->fpart, c->part, c->st, c->pst, c->fst, c->pp, c->dsc, c->x64, c->r64, c->z64,
                     c->pa64, c->pb64, c->q64, c->vface, c->mface};
     for (void *b : bufs)
-        if (b) (void)hipFree(b);
+        if (b && !is_attached(c, b)) (void)hipFree(b);
     for (int l = 0; l < c->nlev; ++l) {
         Level &lv = c->lev[l];
         void *lb[] = {lv.Dg, lv.iD, lv.e, lv.r};
@@ -1615,8 +1735,6 @@ void ia_destroy(ia_ctx *c) {
     for (hipEvent_t e : c->it_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->pin) (void)hipHostFree(c->pin);
-    for (auto &ex : c->fcg_graph)
-        if (ex) (void)hipGraphExecDestroy(ex);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -1629,7 +1747,7 @@ int ia_set_field(ia_ctx *c, int field, const void *host, size_t nbytes) {
     void *b = field_buf(c, field);
     if (!b || n != nbytes) return fail(c, IA_ERR_BADARG, "ia_set_field(%d): %zu bytes, expected %zu", field, nbytes, n);
     HIPCHK(c, hipSetDevice(c->dev));
-    HIPCHK(c, hipMemcpyAsync(b, host, n, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(b, host, n, hipMemcpyDefault, c->s));  // host or device source
     HIPCHK(c, hipStreamSynchronize(c->s));
     return IA_OK;
 }
@@ -1639,7 +1757,7 @@ int ia_get_field(ia_ctx *c, int field, void *host, size_t nbytes) {
     void *b = field_buf(c, field);
     if (!b || n != nbytes) return fail(c, IA_ERR_BADARG, "ia_get_field(%d): %zu bytes, expected %zu", field, nbytes, n);
     HIPCHK(c, hipSetDevice(c->dev));
-    HIPCHK(c, hipMemcpyAsync(host, b, n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipMemcpyAsync(host, b, n, hipMemcpyDefault, c->s));  // host or device destination
     HIPCHK(c, hipStreamSynchronize(c->s));
     return IA_OK;
 }
@@ -1649,6 +1767,38 @@ int ia_field_device_ptr(ia_ctx *c, int field, void **dptr, size_t *nbytes) {
     if (!b) return fail(c, IA_ERR_BADARG, "unknown field %d", field);
     *dptr = b;
     if (nbytes) *nbytes = field_elems(c, field) * (field == IA_FIELD_CBAR ? 1 : c->es);
+    return IA_OK;
+}
+int ia_attach_field(ia_ctx *c, int field, void *dptr, size_t nbytes) {
+    if (!c) return IA_ERR_BADARG;
+    if (!dptr) return fail(c, IA_ERR_BADARG, "ia_attach_field: null buffer");
+    if (!(field == IA_FIELD_F || field == IA_FIELD_F0 || field == IA_FIELD_U || field == IA_FIELD_U0 ||
+          field == IA_FIELD_P))
+        return fail(c, IA_ERR_BADARG, "ia_attach_field: field %d cannot be attached (f, f0, u, u0 or p)", field);
+    size_t need = field_elems(c, field) * c->es;
+    if (nbytes != need) return fail(c, IA_ERR_BADARG, "ia_attach_field: %zu bytes, expected %zu", nbytes, need);
+    HIPCHK(c, hipSetDevice(c->dev));
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, dptr) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != c->dev)
+        return fail(c, IA_ERR_BADARG, "ia_attach_field: not device memory of device %d", c->dev);
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    void *old = nullptr;
+    switch (field) {
+    case IA_FIELD_F:
+    case IA_FIELD_F0: {
+        int &slot = field == IA_FIELD_F ? c->att_fi : c->att_f0i;
+        int q = field == IA_FIELD_F ? c->fi : c->f0i;
+        old = c->fb[q];
+        c->fb[q] = dptr;
+        slot = q;
+        c->ls_valid = false;
+        break;
+    }
+    case IA_FIELD_U: old = c->u; c->u = c->att_u = dptr; break;
+    case IA_FIELD_U0: old = c->u0; c->u0 = c->att_u0 = dptr; break;
+    default: old = c->p; c->p = c->att_p = dptr; c->lev[0].x = dptr; break;
+    }
+    if (old && old != dptr && !is_attached(c, old)) HIPCHK(c, hipFree(old));
     return IA_OK;
 }
 int ia_mg_levels(ia_ctx *c, int l, int *Ng) {

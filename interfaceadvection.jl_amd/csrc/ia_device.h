@@ -17,6 +17,7 @@ struct Grid {
     int per[3];   // periodic flags
     long st[3];   // strides
     long ns;      // cells
+    int exitx;    // WaterLily exitBC: the step's BC! keeps u_x on the exit face (x = Ng[0]-1)
 };
 
 // ---------------------------------------------------------------------------
@@ -98,10 +99,14 @@ __device__ inline int map_per(const Grid &g, int a, int q) {
     if (q == n - 1) return 1;
     return q;
 }
-// WaterLily BC!(a, A, false, perdir) on component c: true -> Dirichlet A[c]
+// WaterLily BC!(a, A, exitBC, perdir) on component c: true -> Dirichlet A[c]
 __device__ inline bool vel_dirichlet(const Grid &g, int c, const int p[3]) {
     int n = g.Ng[c];
-    return !g.per[c] && (p[c] == 0 || p[c] == 1 || p[c] == n - 1);
+    return !g.per[c] && (p[c] == 0 || p[c] == 1 || (p[c] == n - 1 && !(g.exitx && c == 0)));
+}
+// the exit face of u_x under exitBC (left alone by BC!)
+__device__ inline bool exit_face(const Grid &g, int c, const int p[3]) {
+    return g.exitx && c == 0 && p[0] == g.Ng[0] - 1;
 }
 __device__ inline long lin_vel(const Grid &g, int c, const int p[3]) {
     int q[3];
@@ -133,9 +138,15 @@ __device__ inline bool face_map(const Grid &g, int c, const int p[3], int q[3]) 
 }
 
 // write an interior value and every ghost image of it.  kind 0: scalar
-// (BCf!); kind 1: velocity-like component c (BC! with Dirichlet value dval).
+// (BCf!); kind 1: velocity-like component c (BC!(a, A, false) with Dirichlet
+// value dval); kind 2: the step's BC!(u, u_BC, exitBC): as kind 1, except
+// that under exitBC the exit face of u_x (x = Ng[0]-1) takes xval, the value
+// BC! leaves there, and its Neumann images in y and z copy it.
 template <class T>
-__device__ inline void write_images(const Grid &g, T *a, const int p[3], T val, int kind, int c, T dval) {
+__device__ inline void write_images(const Grid &g, T *a, const int p[3], T val, int kind, int c, T dval,
+                                    T xval = T(0)) {
+    const bool xexit = kind == 2 && c == 0 && g.exitx && !g.per[0];
+    if (kind == 2) kind = 1;
     int alt[3][2], na[3];
     bool dir[3][2];
     for (int ax = 0; ax < 3; ++ax) {
@@ -156,7 +167,7 @@ __device__ inline void write_images(const Grid &g, T *a, const int p[3], T val, 
         }
         if (img >= 0) {
             alt[ax][1] = img;
-            dir[ax][1] = vel_axis;
+            dir[ax][1] = vel_axis && !(xexit && ax == 0 && img == n - 1);
             na[ax] = 2;
         }
         // a 1-cell interior (n == 3) has both images
@@ -168,8 +179,9 @@ __device__ inline void write_images(const Grid &g, T *a, const int p[3], T val, 
         for (int i1 = 0; i1 < na[1]; ++i1)
             for (int i0 = 0; i0 < na[0]; ++i0) {
                 bool d = dir[0][i0] || dir[1][i1] || dir[2][i2];
+                bool xe = xexit && i0 == 1 && alt[0][1] == g.Ng[0] - 1;
                 long idx = (long)alt[0][i0] + g.st[1] * alt[1][i1] + g.st[2] * alt[2][i2];
-                a[idx] = d ? dval : val;
+                a[idx] = d ? dval : (xe ? xval : val);
             }
 }
 

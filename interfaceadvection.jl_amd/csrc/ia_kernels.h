@@ -250,7 +250,8 @@ template <class T> __device__ inline void vof_face_one(const SweepArgs<T> &a, T 
     T ff = dl == T(0) ? T(0) : vof_face_flux(a, p, I, dl);
     face[I] = ff;
     T m = dl == T(0) ? T(0) : T(0) + (dl * a.lr + (1 - a.lr) * ff);
-    if (a.mode == 0) m = !g.per[d] && (p[d] == 1 || p[d] == g.Ng[d] - 1) ? a.ubc_d : m * a.idt;
+    if (a.mode == 0)
+        m = !g.per[d] && (p[d] == 1 || (p[d] == g.Ng[d] - 1 && !(g.exitx && d == 0))) ? a.ubc_d : m * a.idt;
     a.ruf[I] = m;
 }
 template <class T>
@@ -380,6 +381,10 @@ template <class T> struct MomArgs {
 
 template <class T> __device__ inline T mom_ustar(const MomArgs<T> &a, int i, const int q[3]) {
     const Grid &g = a.g;
+    // exitBC: BC! leaves uStar's exit face at what the reference's scratch r
+    // (Flow.f) holds there: ρu2u! does not reach it and the forcing left 0
+    // (g_x = 0) or the previous sweep's fill!(r, 0) (flow.jl:197, :215)
+    if (exit_face(g, i, q)) return T(0);
     if (vel_dirichlet(g, i, q)) return a.ubc[i];
     long K = lin_vel(g, i, q);
     return a.ru_in[g.ns * i + K] / linp((a.fpre[K] + a.fpre[K - g.st[i]]) / 2, a.lr);
@@ -405,6 +410,9 @@ __global__ void __launch_bounds__(NT) k_ustar(MomArgs<T> a) {
 // the sweep with the axis d and the dimension as compile-time constants
 template <int DIR, class T> __device__ inline T mom_ruf(const MomArgs<T> &a, const int q[3]) {
     const Grid &g = a.g;
+    // exitBC: ρuf keeps the computed mass flux of the exit face (flow.jl:207)
+    if (DIR == 0 && exit_face(g, 0, q))
+        return a.ruf[(long)q[0] + g.st[1] * map_scalar(g, 1, q[1]) + g.st[2] * map_scalar(g, 2, q[2])];
     if (!g.per[DIR] && (q[DIR] == 0 || q[DIR] == 1 || q[DIR] == g.Ng[DIR] - 1)) return a.ubc[DIR];
     return a.ruf[lin_scalar(g, q[0], q[1], q[2])];
 }
@@ -542,7 +550,8 @@ template <class T> struct ViscArgs {
 template <class T> __device__ inline T visc_ff(const ViscArgs<T> &a, int c, const int q[3]) {
     const Grid &g = a.g;
     int qq[3];
-    if (!face_map(g, c, q, qq)) return a.ubc[c];  // n̂ there still holds uStar = u_BC (flow.jl:206)
+    // n̂ there still holds uStar (flow.jl:206): u_BC, or 0 on the exit face under exitBC
+    if (!face_map(g, c, q, qq)) return exit_face(g, c, q) ? T(0) : a.ubc[c];
     long K = lin(g, qq);
     return (a.fv[K] + a.fv[K - g.st[c]]) / 2;
 }
@@ -854,7 +863,9 @@ __global__ void __launch_bounds__(NT) k_visc_update(UpdArgs<T> a) {
     }
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-        write_images<T>(g, a.uout + g.ns * i, p, un[i], 1, i, a.v.ubc[i]);
+        // exitBC: updateU!'s axpy! (flow.jl:251) is the only update of the exit face
+        T xv = i == 0 && g.exitx ? a.v.u[I + g.st[0]] + (a.dt * wT) * a.g[0] : T(0);
+        write_images<T>(g, a.uout + g.ns * i, p, un[i], 2, i, a.v.ubc[i], xv);
         write_images<T>(g, a.L + g.ns * i, p, T(1) / rh[i], 1, i, T(0));
     }
 }
@@ -948,47 +959,8 @@ __global__ void __launch_bounds__(NT) k_project(Grid g, Tiling t, T *u, const T 
         q[i] -= 1;
         T dx = x[I] - xper(g, x, q[0], q[1], q[2]);
         T un = u[g.ns * i + I] - L[g.ns * i + I] * dx;
-        write_images<T>(g, u + g.ns * i, p, un, 1, i, ub[i]);
-    }
-}
-// MPCFL (flow.jl:262-281): block maxima of flux_out and maxTotalFlux; and,
-// for the step's statistics, the block's FP64 Σf and its min / max f over
-// the interior (5 partials per block)
-template <class T>
-__global__ void __launch_bounds__(NT) k_cfl(Grid g, Tiling t, const T *u, const T *f, double *part) {
-    int p[3];
-    double a = 0, b = 0, fs = 0, fmn = 1e300, fmx = -1e300;
-    if (thread_cell(g, t, p)) {
-        long I = lin(g, p);
-        double acc = 0;
-        T s2 = 0;
-        for (int d = 0; d < g.D; ++d) {
-            double up = (double)u[g.ns * d + I + g.st[d]], lo = -(double)u[g.ns * d + I];
-            acc += (up > 0.0 ? up : 0.0) + (lo > 0.0 ? lo : 0.0);
-            s2 += tmax(fabs(u[g.ns * d + I]), fabs(u[g.ns * d + I + g.st[d]]));
-        }
-        a = (double)(T)acc;
-        b = (double)s2;
-        fs = (double)f[I];
-        fmn = fs;
-        fmx = fs;
-    }
-    double ma = block_max(a);
-    __syncthreads();
-    double mb = block_max(b);
-    __syncthreads();
-    double sf = block_sum(fs);
-    __syncthreads();
-    double mx = block_max(fmx);
-    __syncthreads();
-    double mn = -block_max(-fmn);
-    if (threadIdx.x == 0) {
-        double *o = part + 5 * blockIdx.x;
-        o[0] = ma;
-        o[1] = mb;
-        o[2] = sf;
-        o[3] = mn;
-        o[4] = mx;
+        T xv = i == 0 && g.exitx ? u[I + g.st[0]] : T(0);  // exitBC: the exit face is left alone
+        write_images<T>(g, u + g.ns * i, p, un, 2, i, ub[i], xv);
     }
 }
 // the two CFL maxima, Σf (each thread adds its blocks in order, then a fixed

@@ -25,7 +25,17 @@ constexpr int CNN_NPAR = CNN_L * (CNN_W + CNN_C); // weights + biases
 constexpr int CNN_IN = 3, CNN_OUT = 2;            // real input / output channels
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// gfx950's v_mfma_f32_16x16x32_bf16 contracts 32 = 2 taps × 16 input channels
+// per product: the 27 taps go in CNN_P pairs, the last pair's second tap
+// with zero weights.  Lane l holds, of the pair's K = 32, the eight values
+// k = 8·(l/16) .. +7 of A (row l%16) and of B (column l%16): tap 2p + (l/32),
+// channels 8·((l/16)&1) .. +7.  (The hardware contracts A's and B's k in
+// the same order, so the product sums exactly these 32 terms.)
+constexpr int CNN_P = (CNN_T + 1) / 2;
+__device__ inline bf16x8 ld_bf16x8(const unsigned short *p) { return *(const bf16x8 *)p; }
+__device__ inline int pair_tap(int p, int kg) { return min(2 * p + (kg >> 1), CNN_T - 1); }
 
 __device__ inline unsigned short f2bf(float f) {  // round to nearest even
     unsigned u = __float_as_uint(f);
@@ -54,34 +64,35 @@ __device__ inline long tap_off(const Grid &g, int t) {
 }
 
 // Y = act(conv(X, W) + b) on interior voxels.  Per wave and tile, the 27
-// taps are 27 MFMA 16x16x16 products D[co][vox] += A[co][ci] B[ci][vox] with
-// A the tap's weights (lane l: co = l%16, ci = 4(l/16)..+3) and B the input
-// rows (lane l: voxel l%16, channels 4(l/16)..+3, one 8-byte load); lane l
-// then holds channels 4(l/16)..+3 of voxel l%16 and stores them as 8 bytes.
+// taps are CNN_P MFMA 16x16x32 products D[co][vox] += A[co][k] B[k][vox]
+// over a tap pair, A the pair's weights (lane l: co = l%16) and B the input
+// rows (lane l: voxel l%16, eight channels of its tap, one 16-byte load);
+// lane l then holds channels 4(l/16)..+3 of voxel l%16 and stores them as
+// 8 bytes.
 __global__ void __launch_bounds__(256) k_cnn_conv(CnnTiles t, const unsigned short *__restrict__ X,
                                                   unsigned short *__restrict__ Y,
-                                                  const bf16x4 *__restrict__ Wf,  // [27][64]
+                                                  const bf16x8 *__restrict__ Wf,  // [CNN_P][64]
                                                   const float *__restrict__ bias, int relu) {
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long nwave = ((long)gridDim.x * blockDim.x) >> 6;
-    const int j = lane & 15, kg = lane >> 4;
-    bf16x4 a[CNN_T];
+    const int j = lane & 15, kg = lane >> 4, h8 = 8 * (kg & 1);
+    bf16x8 a[CNN_P];
 #pragma unroll
-    for (int q = 0; q < CNN_T; ++q) a[q] = Wf[q * 64 + lane];
+    for (int p = 0; p < CNN_P; ++p) a[p] = Wf[p * 64 + lane];
     float b[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) b[q] = bias[4 * kg + q];
-    long off[CNN_T];
+    long off[CNN_P];
 #pragma unroll
-    for (int q = 0; q < CNN_T; ++q) off[q] = tap_off(t.g, q);
+    for (int p = 0; p < CNN_P; ++p) off[p] = tap_off(t.g, pair_tap(p, kg));
     for (long tile = wave; tile < t.ntile; tile += nwave) {
         long V = tile_base(t, tile) + j;
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 0; q < CNN_T; ++q) {
-            bf16x4 bx = *(const bf16x4 *)(X + (V + off[q]) * CNN_C + 4 * kg);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bx, acc, 0, 0, 0);
+        for (int p = 0; p < CNN_P; ++p) {
+            bf16x8 bx = ld_bf16x8(X + (V + off[p]) * CNN_C + h8);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bx, acc, 0, 0, 0);
         }
         unsigned short o[4];
 #pragma unroll
@@ -97,8 +108,8 @@ __global__ void __launch_bounds__(256) k_cnn_conv(CnnTiles t, const unsigned sho
 // The same convolution with the input staged through LDS: a block owns a
 // 16 (x) by 4 (y) column of voxels and walks CNN_ZB planes in z.  A ring of
 // four LDS planes holds the input rows y-1..y+4, x-1..x+16 of planes z-1,
-// z, z+1 and the prefetched z+2; every tap's MFMA operand is one 8-byte LDS
-// read, and each input plane is read from memory once per block.
+// z, z+1 and the prefetched z+2; every tap pair's MFMA operand is one
+// 16-byte LDS read, and each input plane is read from memory once per block.
 constexpr int CNN_ZB = 32;
 struct CnnBlocks {
     Grid g;
@@ -106,7 +117,7 @@ struct CnnBlocks {
 };
 __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigned short *__restrict__ X,
                                                       unsigned short *__restrict__ Y,
-                                                      const bf16x4 *__restrict__ Wf, const float *__restrict__ bias,
+                                                      const bf16x8 *__restrict__ Wf, const float *__restrict__ bias,
                                                       int relu) {
     constexpr int PX = 18, PY = 6, PV = PX * PY;  // voxels per staged plane
     __shared__ __align__(16) unsigned short pl[4][PV * CNN_C];
@@ -116,7 +127,7 @@ __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigne
     int x0 = 1 + 16 * bx, y0 = 1 + 4 * by, zlo = 1 + CNN_ZB * bz;
     int zhi = min(zlo + CNN_ZB - 1, g.Ng[2] - 2);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int j = lane & 15, kg = lane >> 4;
+    const int j = lane & 15, kg = lane >> 4, h8 = 8 * (kg & 1);
     // staging: thread s < 216 moves 16 bytes (half a voxel's channels) of a plane
     const int s = threadIdx.x;
     const bool stg = s < 2 * PV;
@@ -129,9 +140,18 @@ __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigne
     auto put = [&](int slot, uint4 v) {
         if (stg) *(uint4 *)(&pl[slot][sv * CNN_C + 8 * sh]) = v;
     };
-    bf16x4 a[CNN_T];
+    bf16x8 a[CNN_P];
 #pragma unroll
-    for (int q = 0; q < CNN_T; ++q) a[q] = Wf[q * 64 + lane];
+    for (int p = 0; p < CNN_P; ++p) a[p] = Wf[p * 64 + lane];
+    // each lane's tap of every pair: the (x, y) offset in the plane and the ring slot shift
+    int loff[CNN_P], lz[CNN_P];
+#pragma unroll
+    for (int p = 0; p < CNN_P; ++p) {
+        int q = pair_tap(p, kg);
+        int dx = q % 3 - 1, dy = (q / 3) % 3 - 1;
+        loff[p] = ((w + 1 + dy) * PX + (j + 1 + dx)) * CNN_C + h8;
+        lz[p] = q / 9 - 1;
+    }
     float bb[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) bb[q] = bias[4 * kg + q];
@@ -144,11 +164,9 @@ __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigne
         if (z + 2 <= zhi + 1) nxt = fetch(z + 2);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 0; q < CNN_T; ++q) {
-            int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
-            const unsigned short *src = &pl[(z + dz) & 3][((w + 1 + dy) * PX + (j + 1 + dx)) * CNN_C + 4 * kg];
-            bf16x4 bx = *(const bf16x4 *)src;
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[q], bx, acc, 0, 0, 0);
+        for (int p = 0; p < CNN_P; ++p) {
+            bf16x8 bx = ld_bf16x8(&pl[(z + lz[p]) & 3][loff[p]]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[p], bx, acc, 0, 0, 0);
         }
         unsigned short o[4];
 #pragma unroll
@@ -163,18 +181,25 @@ __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigne
     }
 }
 
-// MFMA operand order of one layer's weights: forward A[co][ci] per tap, and
-// the transposed convolution's A[ci][co] of the flipped tap
-__global__ void k_cnn_frag(const float *__restrict__ W, bf16x4 *__restrict__ Ff, bf16x4 *__restrict__ Fb) {
-    int t = blockIdx.x, lane = threadIdx.x;  // 27 blocks of 64
-    int j = lane & 15, kg = lane >> 4;
-    unsigned short f[4], r[4];
-    for (int q = 0; q < 4; ++q) {
-        f[q] = f2bf(W[(t * CNN_C + 4 * kg + q) * CNN_C + j]);                 // W[t][ci=4kg+q][co=j]
-        r[q] = f2bf(W[((CNN_T - 1 - t) * CNN_C + j) * CNN_C + 4 * kg + q]);   // W[26-t][ci=j][co=4kg+q]
+// MFMA operand order of one layer's weights, per tap pair p: forward
+// A[co][ci] (lane l: co = l%16, tap 2p + l/32, ci = 8((l/16)&1) .. +7), and
+// the transposed convolution's A[ci][co] of the flipped tap; the missing
+// 28th tap is zero
+__global__ void k_cnn_frag(const float *__restrict__ W, bf16x8 *__restrict__ Ff, bf16x8 *__restrict__ Fb) {
+    int p = blockIdx.x, lane = threadIdx.x;  // CNN_P blocks of 64
+    int j = lane & 15, kg = lane >> 4, h8 = 8 * (kg & 1);
+    int t = 2 * p + (kg >> 1);
+    union {
+        unsigned short u[8];
+        bf16x8 v;
+    } f, r;
+    for (int e = 0; e < 8; ++e) {
+        bool ok = t < CNN_T;
+        f.u[e] = ok ? f2bf(W[(t * CNN_C + h8 + e) * CNN_C + j]) : 0;                  // W[t][ci=h8+e][co=j]
+        r.u[e] = ok ? f2bf(W[((CNN_T - 1 - t) * CNN_C + j) * CNN_C + h8 + e]) : 0;    // W[26-t][ci=j][co=h8+e]
     }
-    Ff[t * 64 + lane] = bf16x4{(short)f[0], (short)f[1], (short)f[2], (short)f[3]};
-    Fb[t * 64 + lane] = bf16x4{(short)r[0], (short)r[1], (short)r[2], (short)r[3]};
+    Ff[p * 64 + lane] = f.v;
+    Fb[p * 64 + lane] = r.v;
 }
 
 // dH = dY ⊙ (H > 0): the ReLU derivative, read from the layer's output

@@ -14,10 +14,15 @@
 
 namespace ia {
 
+// column tiling: a block covers nx (x) by NT/nx (y) columns of zb cells.
+// nx = TX (64) everywhere, except coarse levels of at most 32 cells per row
+// with one cell per thread, where nx = 32 fills every lane (32 by 8).  The
+// LDS-tiled traversals assume nx = TX; they run only where zb > 1.
 struct CTiling {
     int nt[3];
     int zb;
     long nblk;
+    int nx;
 };
 __device__ inline bool thread_col(const Grid &g, const CTiling &t, int p[3], int &zlo, int &zhi) {
     long b = swizzle_block((long)blockIdx.x, t.nblk);
@@ -25,9 +30,9 @@ __device__ inline bool thread_col(const Grid &g, const CTiling &t, int p[3], int
     long r = b / t.nt[0];
     long by = r % t.nt[1];
     long bz = r / t.nt[1];
-    int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-    p[0] = 1 + (int)bx * TX + tx;
-    p[1] = g.D > 1 ? 1 + (int)by * TY + ty : 0;
+    int tx = threadIdx.x % t.nx, ty = threadIdx.x / t.nx;
+    p[0] = 1 + (int)bx * t.nx + tx;
+    p[1] = g.D > 1 ? 1 + (int)by * (NT / t.nx) + ty : 0;
     if (g.D > 2) {
         zlo = 1 + (int)bz * t.zb;
         zhi = zlo + t.zb - 1;
@@ -1093,6 +1098,99 @@ __global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState
     }
 }
 
+// ---------------------------------------------------------------------------
+// psolver! (flow.jl:300-326): Jacobi-preconditioned CG on the single-level
+// Poisson (the finest level's L, D, iD, x, ϵ, r, z), without pcg!'s early
+// exits; the host tests the absolute Σr² after every iteration.  ρ lives in
+// one of two device slots (read one, write the other) so that no block reads
+// a value another block of the same launch is writing.
+// ---------------------------------------------------------------------------
+// ρ₀ = (T) Σ r·z from k_pcg_init's partials (one block)
+template <class T>
+__global__ void __launch_bounds__(NT) k_ps_rho(const double *part, long nblk, double *rho) {
+    T v = (T)block_total(part, nblk);
+    if (threadIdx.x == 0) *rho = (double)v;
+}
+// z = Aϵ, partial z·ϵ
+template <class T>
+__global__ void __launch_bounds__(NT) k_ps_Ae(LevV<T> l, double *partB) {
+    double v = 0;
+    DISPATCH_D(l.g, pcg_Ae_d<D>(l, v);)
+    block_partial(v, partB);
+}
+// α = ρ/(z·ϵ); x += αϵ; r -= αz; z = r·iD; partials r·z and r·r
+template <class T>
+__global__ void __launch_bounds__(NT) k_ps_update(LevV<T> l, const double *rho, const double *partB,
+                                                  double *partC) {
+    T alpha = (T)*rho / (T)block_total(partB, l.t.nblk);
+    double vz = 0, vr = 0;
+    COL_LOOP(l, {
+        l.x[I] = l.x[I] + alpha * l.e[I];
+        T rr = l.r[I] - alpha * l.z[I];
+        l.r[I] = rr;
+        T zz = rr * l.iD[I];
+        l.z[I] = zz;
+        vz += (double)rr * (double)zz;
+        vr += (double)rr * (double)rr;
+    })
+    block_partial(vz, partC, 2, 0);
+    block_partial(vr, partC, 2, 1);
+}
+// ρ₂ = r·z, β = ρ₂/ρ, ϵ = βϵ + z, ρ ← ρ₂ (into the other slot)
+template <class T>
+__global__ void __launch_bounds__(NT) k_ps_eps(LevV<T> l, const double *rho_in, double *rho_out,
+                                               const double *partC) {
+    T rho2 = (T)block_total(partC, l.t.nblk, 2, 0);
+    T beta = rho2 / (T)*rho_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *rho_out = (double)rho2;
+    COL_LOOP(l, { l.e[I] = beta * l.e[I] + l.z[I]; })
+}
+
+// MPCFL (flow.jl:262-281): block maxima of flux_out and maxTotalFlux; and,
+// for the step's statistics, the block's FP64 Σf and its min / max f over
+// the interior (5 partials per block).  Each thread walks a z-column of
+// cells, so the five block reductions run once per column tile.
+template <class T>
+__global__ void __launch_bounds__(NT) k_cfl(Grid g, CTiling t, const T *u, const T *f, double *part) {
+    double a = 0, b = 0, fs = 0, fmn = 1e300, fmx = -1e300;
+    int p[3], zlo, zhi;
+    if (thread_col(g, t, p, zlo, zhi))
+        for (int z = zlo; z <= zhi; ++z) {
+            p[2] = z;
+            long I = (long)p[0] + g.st[1] * p[1] + g.st[2] * p[2];
+            double acc = 0;
+            T s2 = 0;
+            for (int d = 0; d < g.D; ++d) {
+                double up = (double)u[g.ns * d + I + g.st[d]], lo = -(double)u[g.ns * d + I];
+                acc += (up > 0.0 ? up : 0.0) + (lo > 0.0 ? lo : 0.0);
+                s2 += tmax(fabs(u[g.ns * d + I]), fabs(u[g.ns * d + I + g.st[d]]));
+            }
+            double av = (double)(T)acc, bv = (double)s2, fv = (double)f[I];
+            a = av > a ? av : a;
+            b = bv > b ? bv : b;
+            fs += fv;
+            fmn = fv < fmn ? fv : fmn;
+            fmx = fv > fmx ? fv : fmx;
+        }
+    double ma = block_max(a);
+    __syncthreads();
+    double mb = block_max(b);
+    __syncthreads();
+    double sf = block_sum(fs);
+    __syncthreads();
+    double mx = block_max(fmx);
+    __syncthreads();
+    double mn = -block_max(-fmn);
+    if (threadIdx.x == 0) {
+        double *o = part + 5 * blockIdx.x;
+        o[0] = ma;
+        o[1] = mb;
+        o[2] = sf;
+        o[3] = mn;
+        o[4] = mx;
+    }
+}
+
 // perBC!(x): periodic images of the interior
 template <class T>
 __global__ void __launch_bounds__(NT) k_mg_perbc(LevV<T> l, T *x) {
@@ -1559,7 +1657,8 @@ __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, doub
                 T un0 = (T)((double)pv.u0 - (double)pv.l0 * (c - xm));
                 T un1 = (T)((double)pv.u1 - (double)pv.l1 * (c - ym));
                 T un2 = (T)((double)pv.u2 - (double)pv.l2 * (c - zm));
-                write_images<T>(g, u, p, un0, 1, 0, ub[0]);
+                T xv = g.exitx ? u[I + 1] : T(0);  // exitBC: the exit face is left alone
+                write_images<T>(g, u, p, un0, 2, 0, ub[0], xv);
                 write_images<T>(g, u + g.ns, p, un1, 1, 1, ub[1]);
                 write_images<T>(g, u + 2 * g.ns, p, un2, 1, 2, ub[2]);
                 pT[I] = (T)(c / dtw);
@@ -1572,7 +1671,8 @@ __global__ void __launch_bounds__(NT) k_fcg_project(FcgV<T> f, T *u, T *pT, doub
             q[i] -= 1;
             double xm = xper(g, (const double *)f.x, q[0], q[1], q[2]);
             T un = (T)((double)u[g.ns * i + I] - (double)f.L[g.ns * i + I] * (f.x[I] - xm));
-            write_images<T>(g, u + g.ns * i, p, un, 1, i, ub[i]);
+            T xv = i == 0 && g.exitx ? u[I + 1] : T(0);  // exitBC: the exit face is left alone
+            write_images<T>(g, u + g.ns * i, p, un, 2, i, ub[i], xv);
         }
         pT[I] = (T)(f.x[I] / dtw);
     })

@@ -76,6 +76,6 @@ def test_host_rejects_unsupported_plugins():
     with pytest.raises(ValueError):
         ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), η=-1.0)
     with pytest.raises(ValueError):
-        ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), exitBC=True)
+        ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), exitBC=True, g=(0.5, 0.0))
     with pytest.raises(KeyError):
         ia.TwoPhaseSimulation((8, 8), (0.0, 0.0), λ="quick")

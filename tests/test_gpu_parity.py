@@ -196,6 +196,61 @@ def test_advect_single_vortex(T):
     g.close()
 
 
+def channel(N=(48, 24), T=np.float64, exitBC=True):
+    """A channel of the dark fluid with inflow u_BC = (1, 0) at x = 0, the exit
+    at the upper x face (exitBC), slip walls in y (and z), gravity across it,
+    density ratio 1000: a light bubble near the exit, carried out through it."""
+    D = len(N)
+    L = float(N[1])
+    c = [0.85 * N[0], 0.5 * N[1], 0.5 * N[2] if D == 3 else 0.0]
+
+    def sdf(x):  # dark outside the bubble
+        return 0.3 * L - np.sqrt(sum((x[a] - c[a]) ** 2 for a in range(D)))
+
+    uλ = lambda i, x: np.ones_like(x[0]) if i == 1 else np.zeros_like(x[0])
+    g = (0.0, -0.5 / L) + ((0.0,) if D == 3 else ())
+    return dict(dims=N, u_BC=(1.0,) + (0.0,) * (D - 1), L=L,
+                kw=dict(T=T, λρ=1e-3, λμ=1e-2, ν=L / 1e3, g=g, InterfaceSDF=sdf, u0=uλ, exitBC=exitBC))
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_exitbc_steps_match_oracle(D):
+    """exitBC (WaterLily BC! leaving the exit face of u_x alone; flow.jl:69,
+    :79, :82, :197, :207): whole FP64 steps of a light bubble leaving a
+    channel, on the device and the oracle (REFERENCE solve).  The exit face
+    keeps u_x = 1 (only gravity's x component, 0 here, would update it), the
+    inflow faces stay at u_BC, and the oracle's run without exitBC differs by
+    far more than the parity bar (there the exit's mass flux is reset to u_BC,
+    which the light fluid leaving cannot carry), so the exit path is
+    exercised."""
+    say("exitBC channel", f"{D}D")
+    N = (48, 24) if D == 2 else (32, 16, 16)
+    case = channel(N)
+    g, o = pair(case)
+    plain = O.OracleSim(case["dims"], case["u_BC"], case["L"], **dict(case["kw"], exitBC=False))
+    nst = 8 if D == 2 else 4
+    for n in range(nst):
+        ia.sim_step_(g)
+        o.sim_step()
+        plain.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= F_STEP_TOL
+        assert maxdiff(g.flow.u, o.u) <= U_STAG_TOL * max(1.0, np.abs(o.u).max())
+        assert g.flow.Δt[-1] == pytest.approx(o.Δt[-1], rel=1e-12)
+    u = g.flow.u
+    assert np.all(u[-1, ..., 0] == 1.0)          # the exit face of u_x
+    assert np.all(u[0, ..., 0] == 1.0) and np.all(u[1, ..., 0] == 1.0)   # the inflow
+    assert np.abs(u).max() < 1.2
+    assert maxdiff(o.f, plain.f) > 100 * F_STEP_TOL
+    g.close()
+
+
+def test_exitbc_rejects_gravity_along_x():
+    case = channel((32, 16))
+    kw = dict(case["kw"], g=(0.1, -0.01))
+    with pytest.raises(ValueError):
+        ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], **kw)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("T", [np.float64, np.float32])
 def test_divergent_squeeze_explodes(T):
@@ -327,6 +382,62 @@ def test_project_reference(T, per):
     pg, po = O.inside(g.flow.p), O.inside(o.p)
     assert maxdiff(pg - pg.mean(), po - po.mean()) <= tol * max(1.0, np.abs(po).max())
     assert maxdiff(g.flow.u, o.u) <= tol
+    g.close()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("D, per", [(2, ()), (2, (1, 2)), (3, ()), (3, (1, 2, 3))])
+def test_project_psolver(T, D, per):
+    """inproject!(::Poisson) + psolver! (flow.jl:300-341): the single-level
+    Jacobi-PCG to Σr² <= 50eps(T), against the oracle's.  The dot products sum
+    in another order on each side, so the iteration counts may differ by a few
+    (FP64: hundreds of iterations to 1e-14); both stop at the tolerance, and p
+    and u agree to the converged solution's round-off (FP64: 1e-9 relative to
+    max|p|, FP32: 2e-3)."""
+    say("psolver!", T.__name__, f"{D}D", per)
+    case = bubble2d(32, T) if D == 2 else bubbles3d(16, T)
+    case["kw"]["perdir"] = per
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], poisson="pcg", **case["kw"])
+    o = O.OracleSim(case["dims"], case["u_BC"], case["L"], poisson="pcg", **case["kw"])
+    rng = np.random.default_rng(8)
+    N = case["dims"]
+    u = np.zeros(g.Ng + (D,), dtype=T, order="F")
+    u[(slice(1, -1),) * D] = rng.standard_normal(tuple(N) + (D,)) * 0.1
+    g._call("ia_set_initial_velocity", ia._ptr(u), u.nbytes)
+    o.lib.oc_set_initial_velocity(o.ctx, O._ptr(u))
+    g._call("ia_update_L", 0)
+    g._call("ia_mg_update")
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    st = ia.IASolveStats()
+    g._call("ia_project", 0.5, ia.C.byref(st))
+    it, r2 = o.project(0.5)
+    eps = np.finfo(T).eps
+    say(f"iterations gpu {st.iters} oracle {it}, Σr² gpu {st.r2:.3e} oracle {r2:.3e}")
+    assert st.r2 <= 50 * eps and r2 <= 50 * eps
+    assert abs(st.iters - it) <= max(3, it // 20)
+    tol = 1e-9 if T == np.float64 else 2e-3
+    pg, po = O.inside(g.flow.p), O.inside(o.p)
+    assert maxdiff(pg - pg.mean(), po - po.mean()) <= tol * max(1.0, np.abs(po).max())
+    assert maxdiff(g.flow.u, o.u) <= tol
+    g.close()
+
+
+def test_psolver_steps_match_oracle():
+    """Whole MPFMomStep!s with the single-level Poisson (FP64 TGV droplet)."""
+    say("MPFMomStep! with psolver!, TGV droplet FP64")
+    case = tgv_droplet(16, np.float64)
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], poisson="pcg", **case["kw"])
+    o = O.OracleSim(case["dims"], case["u_BC"], case["L"], poisson="pcg", **case["kw"])
+    for _ in range(3):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= F_STEP_TOL
+        assert maxdiff(g.flow.u, o.u) <= 1e-9 * max(1.0, np.abs(o.u).max())
+        # psolver! stops at Σr² ≤ 50eps, where the two sides' dot products
+        # (summed in different orders) differ in the last bits: u agrees to
+        # 1e-9 relative, and so does the CFL Δt computed from it.
+        assert g.flow.Δt[-1] == pytest.approx(o.Δt[-1], rel=1e-9)
     g.close()
 
 
