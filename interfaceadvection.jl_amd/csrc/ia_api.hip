@@ -958,6 +958,13 @@ template <class T> struct Impl {
         t.ntile = (long)t.tx * (t.g.Ng[1] - 2) * (t.g.Ng[2] - 2);
         return t;
     }
+    static CnnSegs cnn_segs(ia_ctx *c) {
+        CnnSegs t;
+        t.g = c->lev[0].g;
+        t.sx = (t.g.Ng[0] - 2 + WG_SEG - 1) / WG_SEG;
+        t.nseg = (long)t.sx * (t.g.Ng[1] - 2) * (t.g.Ng[2] - 2);
+        return t;
+    }
     static dim3 cnn_blocks(const CnnTiles &t) { return dim3((unsigned)std::min<long>(2048, (t.ntile + 3) / 4)); }
     static int cnn_alloc(ia_ctx *c) {
         auto &n = c->cnn;
@@ -983,8 +990,7 @@ template <class T> struct Impl {
         HIPCHK(c, hipMemsetAsync(n.zbias, 0, CNN_C * sizeof(float), c->s));
         HIPCHK(c, hipMalloc(&n.ff, CNN_L * CNN_P * 64 * sizeof(bf16x8)));
         HIPCHK(c, hipMalloc(&n.fb, CNN_L * CNN_P * 64 * sizeof(bf16x8)));
-        CnnTiles t = cnn_tiles(c);
-        n.wnblk = std::min<long>(1024, t.ntile);
+        n.wnblk = std::min<long>(1024, cnn_segs(c).nseg);
         HIPCHK(c, hipMalloc((void **)&n.wpart, n.wnblk * (CNN_W + CNN_C) * sizeof(float)));
         return IA_OK;
     }
@@ -1003,16 +1009,18 @@ template <class T> struct Impl {
         return t;
     }
     // Y = act(conv(X) + b): the LDS-staged kernel where rows of 4 fit, else the direct one
-    static void cnn_conv(ia_ctx *c, const void *X, void *Y, const void *frag, const float *bias, int relu) {
+    static void cnn_conv(ia_ctx *c, const void *X, void *Y, const void *frag, const float *bias, int relu,
+                         const void *mask = nullptr) {
         Grid &g = c->lev[0].g;
         if ((g.Ng[1] - 2) % 4 == 0) {
             CnnBlocks t = cnn_lds_blocks(c);
             hipLaunchKernelGGL(k_cnn_conv_lds, dim3((unsigned)((long)t.tx * t.ty * t.tz)), dim3(256), 0, c->s, t,
-                               (const unsigned short *)X, (unsigned short *)Y, (const bf16x8 *)frag, bias, relu);
+                               (const unsigned short *)X, (unsigned short *)Y, (const bf16x8 *)frag, bias, relu,
+                               (const unsigned short *)mask);
         } else {
             CnnTiles t = cnn_tiles(c);
             hipLaunchKernelGGL(k_cnn_conv, cnn_blocks(t), dim3(256), 0, c->s, t, (const unsigned short *)X,
-                               (unsigned short *)Y, (const bf16x8 *)frag, bias, relu);
+                               (unsigned short *)Y, (const bf16x8 *)frag, bias, relu, (const unsigned short *)mask);
         }
     }
     static void cnn_forward(ia_ctx *c) {
@@ -1024,17 +1032,16 @@ template <class T> struct Impl {
     // gradients of every parameter into gw, from the output gradient in grad[0]
     static void cnn_backward(ia_ctx *c) {
         auto &n = c->cnn;
-        CnnTiles t = cnn_tiles(c);
+        CnnSegs sg = cnn_segs(c);
         int cur = 0;
         for (int l = CNN_L - 1; l >= 0; --l) {
-            hipLaunchKernelGGL(k_cnn_wgrad, dim3((unsigned)n.wnblk), dim3(256), 0, c->s, t,
+            hipLaunchKernelGGL(k_cnn_wgrad, dim3((unsigned)n.wnblk), dim3(256), 0, c->s, sg,
                                (const unsigned short *)n.act[l], (const unsigned short *)n.grad[cur], n.wpart);
-            hipLaunchKernelGGL(k_cnn_wgrad_sum, dim3((CNN_W + CNN_C + 255) / 256), dim3(256), 0, c->s, n.wpart, n.wnblk,
+            hipLaunchKernelGGL(k_cnn_wgrad_sum, dim3((CNN_W + CNN_C + 15) / 16), dim3(256), 0, c->s, n.wpart, n.wnblk,
                                n.gw + (long)l * CNN_W, n.gw + (long)CNN_L * CNN_W + l * CNN_C);
             if (l == 0) break;
-            cnn_conv(c, n.grad[cur], n.grad[1 - cur], (const bf16x8 *)n.fb + l * CNN_P * 64, n.zbias, 0);
-            hipLaunchKernelGGL(k_cnn_relu_bwd, dim3((unsigned)((t.ntile * 64 + 255) / 256)), dim3(256), 0, c->s, t,
-                               (unsigned short *)n.grad[1 - cur], (const unsigned short *)n.act[l]);
+            // the transposed convolution, masked by the ReLU derivative of layer l's input
+            cnn_conv(c, n.grad[cur], n.grad[1 - cur], (const bf16x8 *)n.fb + l * CNN_P * 64, n.zbias, 0, n.act[l]);
             cur ^= 1;
         }
     }

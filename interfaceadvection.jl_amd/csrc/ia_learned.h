@@ -63,7 +63,19 @@ __device__ inline long tap_off(const Grid &g, int t) {
     return (long)dx + g.st[1] * dy + g.st[2] * dz;
 }
 
-// Y = act(conv(X, W) + b) on interior voxels.  Per wave and tile, the 27
+// the ReLU derivative of the backward pass, fused into the transposed
+// convolution's store: dH = dY ⊙ (H > 0) with H the layer's input (mask)
+__device__ inline void cnn_mask(const unsigned short *__restrict__ mask, long o, unsigned short v[4]) {
+    if (!mask) return;
+    uint2 h = *(const uint2 *)(mask + o);
+    unsigned short hv[4] = {(unsigned short)(h.x & 0xffffu), (unsigned short)(h.x >> 16),
+                            (unsigned short)(h.y & 0xffffu), (unsigned short)(h.y >> 16)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (!(bf2f(hv[q]) > 0.f)) v[q] = 0;
+}
+
+// Y = act(conv(X, W) + b) on interior voxels (then masked: see cnn_mask).  Per wave and tile, the 27
 // taps are CNN_P MFMA 16x16x32 products D[co][vox] += A[co][k] B[k][vox]
 // over a tap pair, A the pair's weights (lane l: co = l%16) and B the input
 // rows (lane l: voxel l%16, eight channels of its tap, one 16-byte load);
@@ -72,7 +84,8 @@ __device__ inline long tap_off(const Grid &g, int t) {
 __global__ void __launch_bounds__(256) k_cnn_conv(CnnTiles t, const unsigned short *__restrict__ X,
                                                   unsigned short *__restrict__ Y,
                                                   const bf16x8 *__restrict__ Wf,  // [CNN_P][64]
-                                                  const float *__restrict__ bias, int relu) {
+                                                  const float *__restrict__ bias, int relu,
+                                                  const unsigned short *__restrict__ mask) {
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long nwave = ((long)gridDim.x * blockDim.x) >> 6;
@@ -101,6 +114,7 @@ __global__ void __launch_bounds__(256) k_cnn_conv(CnnTiles t, const unsigned sho
             if (relu && v < 0.f) v = 0.f;
             o[q] = f2bf(v);
         }
+        cnn_mask(mask, V * CNN_C + 4 * kg, o);
         *(uint2 *)(Y + V * CNN_C + 4 * kg) = make_uint2(o[0] | ((unsigned)o[1] << 16), o[2] | ((unsigned)o[3] << 16));
     }
 }
@@ -118,7 +132,7 @@ struct CnnBlocks {
 __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigned short *__restrict__ X,
                                                       unsigned short *__restrict__ Y,
                                                       const bf16x8 *__restrict__ Wf, const float *__restrict__ bias,
-                                                      int relu) {
+                                                      int relu, const unsigned short *__restrict__ mask) {
     constexpr int PX = 18, PY = 6, PV = PX * PY;  // voxels per staged plane
     __shared__ __align__(16) unsigned short pl[4][PV * CNN_C];
     const Grid &g = t.g;
@@ -176,6 +190,7 @@ __global__ void __launch_bounds__(256) k_cnn_conv_lds(CnnBlocks t, const unsigne
             o[q] = f2bf(v);
         }
         long V = (long)(x0 + j) + g.st[1] * (y0 + w) + g.st[2] * z;
+        cnn_mask(mask, V * CNN_C + 4 * kg, o);
         *(uint2 *)(Y + V * CNN_C + 4 * kg) = make_uint2(o[0] | ((unsigned)o[1] << 16), o[2] | ((unsigned)o[3] << 16));
         if (z + 2 <= zhi + 1) put((z + 2) & 3, nxt);
     }
@@ -202,65 +217,149 @@ __global__ void k_cnn_frag(const float *__restrict__ W, bf16x8 *__restrict__ Ff,
     Fb[p * 64 + lane] = r.v;
 }
 
-// dH = dY ⊙ (H > 0): the ReLU derivative, read from the layer's output
-__global__ void __launch_bounds__(256) k_cnn_relu_bwd(CnnTiles t, unsigned short *__restrict__ dY,
-                                                      const unsigned short *__restrict__ H) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (tile, lane): 16 voxels x 16 ch
-    if (i >= t.ntile * 64) return;
-    long tile = i >> 6;
-    int lane = (int)(i & 63), j = lane & 15, kg = lane >> 4;
-    long o = (tile_base(t, tile) + j) * CNN_C + 4 * kg;
-    for (int q = 0; q < 4; ++q)
-        if (!(bf2f(H[o + q]) > 0.f)) dY[o + q] = 0;
-}
-
 // weight and bias gradient partials of one layer:
 // gW[t][ci][co] = Σ_v X[v+off_t][ci] dY[v][co], gb[co] = Σ_v dY[v][co].
-// Thread (ci, co) of a block sums over the block's tiles with the tile's
-// inputs and output gradients staged in LDS; partials per block.
-__global__ void __launch_bounds__(256) k_cnn_wgrad(CnnTiles t, const unsigned short *__restrict__ X,
+// Each tap is a 16x16 product over voxels, v_mfma_f32_16x16x32_bf16 with
+// A[ci][v] = X[v+off_t][ci] and B[v][co] = dY[v][co] over 32 voxels (one
+// x-row segment).  The segment's nine input rows (y-1..y+1, z-1..z+1, x-1 ..
+// x+32) and its output gradients are staged in LDS channel-major, the input
+// rows once per x-shift (-1, 0, +1), so that a lane's eight voxels of one
+// channel are one aligned 16-byte read.  Wave w accumulates taps w, w+4, ...;
+// wave 3's last slot multiplies dY by ones, which is the bias gradient.  A
+// product of two BF16 values is exact in FP32; the sums run in FP32 per
+// block, then over the blocks in FP64 (k_cnn_wgrad_sum).  The next
+// segment's loads are issued before the current one's products.
+constexpr int WG_SEG = 32, WG_ROW = 40, WG_K = 7;  // voxels per segment, LDS row (padded), slots per wave
+struct CnnSegs {
+    Grid g;
+    int sx;      // segments per x-row: ceil(N0 / 32)
+    long nseg;   // sx * N1 * N2
+};
+__device__ inline void wg_unpack(uint4 v, unsigned short u[8]) {
+    unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        u[2 * k] = (unsigned short)(w[k] & 0xffffu);
+        u[2 * k + 1] = (unsigned short)(w[k] >> 16);
+    }
+}
+__global__ void __launch_bounds__(256) k_cnn_wgrad(CnnSegs t, const unsigned short *__restrict__ X,
                                                    const unsigned short *__restrict__ dY,
                                                    float *__restrict__ part /* [nblk][27*256 + 16] */) {
-    __shared__ float xs[3][3][18][CNN_C];  // input rows y-1..y+1, z-1..z+1, x0-1..x0+16
-    __shared__ float ds[16][CNN_C];
-    const int ci = threadIdx.x >> 4, co = threadIdx.x & 15;
-    float acc[CNN_T];
-#pragma unroll
-    for (int q = 0; q < CNN_T; ++q) acc[q] = 0.f;
-    float accb = 0.f;
+    __shared__ __align__(16) unsigned short xt[9][3][CNN_C][WG_ROW];
+    __shared__ __align__(16) unsigned short dyt[CNN_C][WG_ROW];
+    constexpr int NST = 9 * (WG_SEG + 2) * 2;  // staged 16-byte pieces of the input rows
+    constexpr int NIT = (NST + 255) / 256;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = lane & 15, kg = lane >> 4;
     const Grid &g = t.g;
-    for (long tile = blockIdx.x; tile < t.ntile; tile += gridDim.x) {
-        long B0 = tile_base(t, tile);
-        __syncthreads();
-        for (int e = threadIdx.x; e < 9 * 18 * CNN_C; e += 256) {
-            int c = e % CNN_C, xx = (e / CNN_C) % 18, yz = e / (CNN_C * 18);
-            int yy = yz % 3, zz = yz / 3;
-            long V = B0 + (xx - 1) + g.st[1] * (yy - 1) + g.st[2] * (zz - 1);
-            xs[zz][yy][xx][c] = bf2f(X[V * CNN_C + c]);
-        }
-        for (int e = threadIdx.x; e < 16 * CNN_C; e += 256) ds[e / CNN_C][e % CNN_C] = bf2f(dY[(B0 + e / CNN_C) * CNN_C + e % CNN_C]);
-        __syncthreads();
-        for (int v = 0; v < 16; ++v) {
-            float d = ds[v][co];
-            accb += d;
+    const int nx = g.Ng[0] - 2;
+    f32x4 acc[WG_K];
 #pragma unroll
-            for (int q = 0; q < CNN_T; ++q) acc[q] += xs[q / 9][(q / 3) % 3][v + q % 3][ci] * d;
+    for (int k = 0; k < WG_K; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const unsigned short one[8] = {0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80};  // BF16 1.0
+    const bf16x8 ones = ld_bf16x8(one);
+    uint4 xv[NIT], dv = make_uint4(0u, 0u, 0u, 0u);
+    auto fetch = [&](long s) {
+        long r = s / t.sx;
+        int x0 = 1 + WG_SEG * (int)(s % t.sx);
+        int y = 1 + (int)(r % (g.Ng[1] - 2)), z = 1 + (int)(r / (g.Ng[1] - 2));
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            int e = threadIdx.x + 256 * it;
+            xv[it] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < NST) {
+                int h = e & 1, vv = (e >> 1) % (WG_SEG + 2), row = (e >> 1) / (WG_SEG + 2);
+                int x = x0 - 1 + vv;
+                if (x <= g.Ng[0] - 1) {
+                    long V = (long)x + g.st[1] * (y + row % 3 - 1) + g.st[2] * (z + row / 3 - 1);
+                    xv[it] = *(const uint4 *)(X + V * CNN_C + 8 * h);
+                }
+            }
+        }
+        dv = make_uint4(0u, 0u, 0u, 0u);
+        if (threadIdx.x < 2 * WG_SEG) {
+            int h = threadIdx.x & 1, v = threadIdx.x >> 1;
+            if (x0 + v <= nx) {
+                long V = (long)(x0 + v) + g.st[1] * y + g.st[2] * z;
+                dv = *(const uint4 *)(dY + V * CNN_C + 8 * h);
+            }
+        }
+    };
+    long s = blockIdx.x;
+    if (s < t.nseg) fetch(s);
+    for (; s < t.nseg; s += gridDim.x) {
+        __syncthreads();  // the previous segment's products have read the LDS
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            int e = threadIdx.x + 256 * it;
+            if (e < NST) {
+                int h = e & 1, vv = (e >> 1) % (WG_SEG + 2), row = (e >> 1) / (WG_SEG + 2);
+                unsigned short u[8];
+                wg_unpack(xv[it], u);
+#pragma unroll
+                for (int sh = 0; sh < 3; ++sh) {
+                    int v = vv - sh;  // shift sh holds X at x0 + v + sh - 1
+                    if (v >= 0 && v < WG_SEG) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) xt[row][sh][8 * h + j][v] = u[j];
+                    }
+                }
+            }
+        }
+        if (threadIdx.x < 2 * WG_SEG) {
+            int h = threadIdx.x & 1, v = threadIdx.x >> 1;
+            unsigned short u[8];
+            wg_unpack(dv, u);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dyt[8 * h + j][v] = u[j];
+        }
+        __syncthreads();
+        if (s + gridDim.x < t.nseg) fetch(s + gridDim.x);
+        const bf16x8 b = ld_bf16x8(&dyt[m][8 * kg]);
+#pragma unroll
+        for (int k = 0; k < WG_K; ++k) {
+            int q = w + 4 * k;  // tap q: row q / 3 = (dz+1)*3 + (dy+1), shift q % 3 = dx + 1
+            if (q < CNN_T) {
+                const bf16x8 a = ld_bf16x8(&xt[q / 3][q % 3][m][8 * kg]);
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[k], 0, 0, 0);
+            } else if (q == CNN_T) {
+                acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ones, b, acc[k], 0, 0, 0);
+            }
         }
     }
+    // lane l holds D[ci = 4(l/16) + j][co = l%16]
     float *P = part + (long)blockIdx.x * (CNN_W + CNN_C);
 #pragma unroll
-    for (int q = 0; q < CNN_T; ++q) P[(q * CNN_C + ci) * CNN_C + co] = acc[q];
-    if (ci == 0) P[CNN_W + co] = accb;  // thread (0, co) saw every voxel's dY[co] (ci plays no part)
+    for (int k = 0; k < WG_K; ++k) {
+        int q = w + 4 * k;
+        if (q < CNN_T) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) P[(q * CNN_C + 4 * kg + j) * CNN_C + m] = acc[k][j];
+        } else if (q == CNN_T && kg == 0) {
+            P[CNN_W + m] = acc[k][0];  // every row of the ones product is Σ_v dY[v][co]
+        }
+    }
 }
-// fixed-order sum of the per-block partials
+// fixed-order sum of the per-block partials: a block owns 16 outputs, its
+// thread (o, k) sums blocks k, k+16, ... of output o in FP64, and thread
+// (o, 0) adds the 16 group sums in order
 __global__ void __launch_bounds__(256) k_cnn_wgrad_sum(const float *__restrict__ part, long nblk,
                                                        float *__restrict__ gW, float *__restrict__ gb) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= CNN_W + CNN_C) return;
+    __shared__ double sh[16][17];
+    const int o = threadIdx.x & 15, k = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + o;
     double s = 0;
-    for (long b = 0; b < nblk; ++b) s += part[b * (CNN_W + CNN_C) + i];
-    if (i < CNN_W) gW[i] = (float)s;
-    else gb[i - CNN_W] = (float)s;
+    if (i < CNN_W + CNN_C)
+        for (long b = k; b < nblk; b += 16) s += part[b * (CNN_W + CNN_C) + i];
+    sh[k][o] = s;
+    __syncthreads();
+    if (k == 0 && i < CNN_W + CNN_C) {
+        double t = 0;
+        for (int q = 0; q < 16; ++q) t += sh[q][o];
+        if (i < CNN_W) gW[i] = (float)t;
+        else gb[i - CNN_W] = (float)t;
+    }
 }
 
 // Adam on every parameter; frozen entries (inputs that are always zero,
