@@ -441,6 +441,28 @@ def test_psolver_steps_match_oracle():
     g.close()
 
 
+@pytest.mark.parametrize("D", [2, 3])
+def test_psolver_walls_steps_match_oracle(D):
+    """psolver! with slip walls, gravity and density ratio 1000 (FP64): two
+    whole steps of bubbles under a free surface.  psolver! stops at
+    Σr² ≤ 50eps, where one iteration more or less (the two sides sum their dot
+    products in different orders) moves u by about 1e-9 of max|u| and f by
+    about 1e-9 (measured on the oracle alone, perturbing the TGV droplet's
+    initial u by 1e-15 relative): bars 1e-8."""
+    say("MPFMomStep! with psolver!, walls + gravity FP64", f"{D}D")
+    case = bubble2d(24, np.float64) if D == 2 else bubbles3d(16, np.float64)
+    g = ia.TwoPhaseSimulation(case["dims"], case["u_BC"], case["L"], poisson="pcg", **case["kw"])
+    o = O.OracleSim(case["dims"], case["u_BC"], case["L"], poisson="pcg", **case["kw"])
+    for _ in range(2):
+        ia.sim_step_(g)
+        o.sim_step()
+        assert maxdiff(g.intf.f, o.f) <= 1e-8
+        assert maxdiff(g.flow.u, o.u) <= 1e-8 * max(1.0, np.abs(o.u).max())
+        assert g.flow.Δt[-1] == pytest.approx(o.Δt[-1], rel=1e-8)
+    assert np.abs(o.u).max() > 0  # gravity has moved the fluid
+    g.close()
+
+
 @pytest.mark.parametrize("per", [(), (1, 2, 3)])
 def test_project_reference_3d_tiled(per):
     """WaterLily's solver! in FP64 at 96^3: level 0 takes the LDS-tiled
