@@ -357,6 +357,86 @@ template <class T> __device__ inline void normal_wh(const Grid &g, const T *f, l
         }
     }
 }
+// normal_wh of a 3D cell from its 3x3x3 block, loaded in one round instead
+// of WH's three dependent rounds (the central differences, then the column
+// heights along the dominant axis, then the row widths).  Every value and
+// operation is the one normal_wh uses, in the same order, so the result is
+// identical; the axis choices select among column sums held in registers.
+template <class T> __device__ inline T sel3(T v0, T v1, T v2, int k) { return k == 0 ? v0 : (k == 1 ? v1 : v2); }
+template <class T> __device__ inline void normal_wh_block3(const Grid &g, const T *f, long I, T *n) {
+    T b[3][3][3];  // b[dz+1][dy+1][dx+1] = f at I + (dx, dy, dz)
+#pragma unroll
+    for (int z = 0; z < 3; ++z)
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) b[z][y][x] = f[I + (x - 1) + g.st[1] * (y - 1) + g.st[2] * (z - 1)];
+    // Hm[d][s] = h3 at I - e_d along s, Hp[d][s] = h3 at I + e_d along s (d != s), Cs[s] = h3 at I along s
+    T Hm[3][3], Hp[3][3], Cs[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int es0 = s == 0, es1 = s == 1, es2 = s == 2;
+        Cs[s] = (b[1][1][1] + b[1 - es2][1 - es1][1 - es0]) + b[1 + es2][1 + es1][1 + es0];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d == s) {
+                Hm[d][s] = Hp[d][s] = T(0);
+                continue;
+            }
+            const int o0 = d == 0, o1 = d == 1, o2 = d == 2;
+            Hm[d][s] = (b[1 - o2][1 - o1][1 - o0] + b[1 - o2 - es2][1 - o1 - es1][1 - o0 - es0]) +
+                       b[1 - o2 + es2][1 - o1 + es1][1 - o0 + es0];
+            Hp[d][s] = (b[1 + o2][1 + o1][1 + o0] + b[1 + o2 - es2][1 + o1 - es1][1 + o0 - es0]) +
+                       b[1 + o2 + es2][1 + o1 + es1][1 + o0 + es0];
+        }
+    }
+    // normal_column
+    n[0] = b[1][1][0] - b[1][1][2];
+    n[1] = b[1][0][1] - b[1][2][1];
+    n[2] = b[0][1][1] - b[2][1][1];
+    int dom = argabsmax(3, n);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d == dom) {
+            T sg = tsign(n[d]);
+            n[d] = sg == T(0) ? T(1) : sg;
+            continue;
+        }
+        T hl = sel3(Hm[d][0], Hm[d][1], Hm[d][2], dom), hr = sel3(Hp[d][0], Hp[d][1], Hp[d][2], dom);
+        n[d] = (hl - hr) / 2;
+    }
+    // normal_wh
+    int im = argabsmax(3, n);
+    T nim = sel3(n[0], n[1], n[2], im);
+    int sdom = signbit(nim) ? -1 : 1;
+    T an = fabs(nim);
+    an = an == T(0) ? T(1) : an;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) n[d] /= an;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d == im) continue;
+        T slope = fabs(n[d]);
+        int scur = signbit(n[d]) ? -1 : 1;
+        T hm = sel3(Hm[d][0], Hm[d][1], Hm[d][2], im), hp = sel3(Hp[d][0], Hp[d][1], Hp[d][2], im);
+        T hl = scur > 0 ? hm : hp, hc = sel3(Cs[0], Cs[1], Cs[2], im), hr = scur > 0 ? hp : hm;
+        T sh = (hl + hc) + hr;
+        double s45 = 4.5 * (double)slope;
+        if (s45 <= (double)sh && (double)sh <= 9 - s45) continue;
+        double r1 = (double)(T(1) / (2 * slope));
+        double mn = s45 < r1 ? s45 : r1;
+        // columns along d at I -+ sdom e_im
+        T cm = sel3(Hm[0][d], Hm[1][d], Hm[2][d], im), cp = sel3(Hp[0][d], Hp[1][d], Hp[2][d], im);
+        if ((double)sh < s45) {
+            T wb = sdom > 0 ? cm : cp;
+            if ((double)wb > mn) n[d] = (T)copysign(((double)hl - 0.5) / ((double)wb - 0.5), (double)scur);
+        }
+        if ((double)sh > 9 - s45) {
+            T wt = sdom > 0 ? cp : cm;
+            if ((double)wt < 3 - mn) n[d] = (T)copysign((2.5 - (double)hr) / (2.5 - (double)wt), (double)scur);
+        }
+    }
+}
 // normalEstimation.jl:35-68 (WY)
 template <class T> __device__ inline void normal_wy(const Grid &g, const T *f, long I, T *n) {
     for (int d = 0; d < g.D; ++d) n[d] = f[I - g.st[d]] - f[I + g.st[d]];

@@ -230,7 +230,8 @@ __device__ inline T vof_face_flux(const SweepArgs<T> &a, const int q[3], long IF
         ICr = lin(g, cc);
     }
     T n[3];
-    normal_scheme(a.scheme, g, a.fin, ICr, n);
+    if (g.D == 3 && a.scheme == 0) normal_wh_block3(g, a.fin, ICr, n);
+    else normal_scheme(a.scheme, g, a.fin, ICr, n);
     T al = getIntercept(g.D, n, fc);
     T aa = dl > 0 ? al - n[d] * (1 - dl) : al;
     T nc[3];
