@@ -107,6 +107,49 @@ def test_pool64_refined_steps_match_oracle(smooth):
     g.close()
 
 
+def test_refined_steps_non_cubic_match_oracle():
+    """The benchmarked REFINED step on a grid of unequal sides, 96 x 48 x 64:
+    the level-0 tiles are half filled along x, the column tilings differ per
+    axis and level, and the coarse levels reach the 32-wide rows and the tail
+    at different sizes per axis.  Three bubbles under a free surface at 0.7 of
+    the height, slip walls, gravity -z, density ratio 1000; two whole steps
+    against the FP32 oracle, with the 64³ pool test's bars."""
+    dims = (96, 48, 64)
+    say("REFINED whole steps on", dims)
+    H = float(dims[2])
+    bub = [(30.0, 20.0, 20.0, 7.0), (62.0, 28.0, 30.0, 5.5), (75.0, 15.0, 14.0, 4.0)]
+
+    def sdf(x):
+        s = x[2] - 0.7 * H
+        for (cx, cy, cz, r) in bub:
+            s = np.maximum(s, r - np.sqrt((x[0] - cx) ** 2 + (x[1] - cy) ** 2 + (x[2] - cz) ** 2))
+        return s
+
+    kw = dict(T=np.float32, λρ=1e-3, λμ=1e-2, ν=H / 1e4, g=(0.0, 0.0, -1.0 / H), InterfaceSDF=sdf)
+    g = ia.TwoPhaseSimulation(dims, (0.0, 0.0, 0.0), H, poisson="refined", poisson_rtol=1e-6,
+                              poisson_max_outer=200, **kw)
+    o = O.OracleSim(dims, (0.0, 0.0, 0.0), H, poisson="refined", poisson_rtol=1e-6, **kw)
+    assert float(np.abs(g.intf.f.astype(np.float64) - o.f.astype(np.float64)).max()) <= 1e-6
+    o.set(O.F, g.intf.f)
+    o.set(O.F0, g.intf.f0)
+    o.Δt = [o.T.type(g.flow.Δt[-1])]
+    for n in range(2):
+        st = ia.sim_step_(g)
+        o.sim_step()
+        gi = [st.solve[0].iters, st.solve[1].iters]
+        oi = [o.lib.oc_last_iters(o.ctx, 0), o.lib.oc_last_iters(o.ctx, 1)]
+        df = float(np.abs(g.intf.f.astype(np.float64) - o.f.astype(np.float64)).max())
+        du = rel_u(g.flow.u, o.u)
+        say(f"step {n + 1}: max|Δf| {df:.3e}, max|Δu|/max|u| {du:.3e}, FCG its gpu {gi} oracle {oi}")
+        assert st.status == 0
+        assert df <= 1e-6
+        assert du <= 1e-5
+        for q in range(2):
+            assert 0 < st.solve[q].rel <= 1e-6
+            assert abs(gi[q] - oi[q]) <= 1
+    g.close()
+
+
 def test_pool64_fp32_error_vs_fp64_oracle():
     say("pool 64^3 REFINED FP32 GPU vs FP64 oracle, 5 steps")
     g, o = pool_pair(64, np.float64)
