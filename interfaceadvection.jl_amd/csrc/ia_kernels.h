@@ -213,14 +213,16 @@ template <class T> struct SweepArgs {
     T dt, lr, idt, rtol, ctol, ubc_d;
 };
 
+// fup, fdn: f at the face's upper and lower cells, loaded with the face
+// velocities (the donor is known only once δl's sign is)
 template <class T>
-__device__ inline T vof_face_flux(const SweepArgs<T> &a, const int q[3], long IF, T dl) {
+__device__ inline T vof_face_flux(const SweepArgs<T> &a, const int q[3], long IF, T dl, T fup, T fdn) {
     const Grid &g = a.g;
     int d = a.d;
     int c[3] = {q[0], q[1], q[2]};
     if (dl > 0) c[d] -= 1;
     long IC = IF - (dl > 0 ? g.st[d] : 0);
-    T fc = a.fin[IC];
+    T fc = dl > 0 ? fdn : fup;
     bool ghost = c[d] == 0 || c[d] == g.Ng[d] - 1;
     if (fullorempty(fc) || (ghost && !g.per[d])) return fc * dl;
     long ICr = IC;
@@ -247,8 +249,9 @@ template <class T> __device__ inline void vof_face_one(const SweepArgs<T> &a, T 
                                                        long I) {
     const Grid &g = a.g;
     int d = a.d;
+    T fup = a.fin[I], fdn = a.fin[I - g.st[d]];
     T dl = (a.dt / 2) * (a.u1[I] + a.u2[I]);
-    T ff = dl == T(0) ? T(0) : vof_face_flux(a, p, I, dl);
+    T ff = dl == T(0) ? T(0) : vof_face_flux(a, p, I, dl, fup, fdn);
     face[I] = ff;
     T m = dl == T(0) ? T(0) : T(0) + (dl * a.lr + (1 - a.lr) * ff);
     if (a.mode == 0)
