@@ -17,8 +17,8 @@ what the mathematics fixes (CPU only, FP64 oracle):
   p_j - p_{j-1} = ρ_face g_y on every vertical face.
 - updateU! in a periodic box (flow.jl:244-252, :74-77): free fall, u = gΔt,
   and the interface translated by the predictor's gΔt²/2.
-- WaterLily BC!(a, A, exitBC, perdir) (Appendix of the survey; flow.jl:69,
-  :79, :82, :197, :207): with exitBC the exit face of u_x (the upper x face)
+- WaterLily BC!(a, A, exitBC, perdir) (as flow.jl:69, :79, :82, :197, :207
+  call it): with exitBC the exit face of u_x (the upper x face)
   keeps its value, and the tangential Neumann images at that face copy it;
   everything else is as without.
 - reportFillError (advection.jl:145-189): a strongly divergent velocity at the
