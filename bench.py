@@ -241,8 +241,8 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (the path shards by z-slabs across GPUs by design, "
-              "but no multi-GPU run is part of it)", file=sys.stderr)
+        print("bench.py: runs on one GPU only (the path would shard by z-slabs across GPUs, "
+              "but no multi-GPU path is built)", file=sys.stderr)
         sys.exit(2)
     ia = load_package()
     case = dict(pool.CASES[args.case])
