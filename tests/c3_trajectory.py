@@ -106,9 +106,11 @@ def oracle(dev, out, threads):
         rec["vol"].append(float(O.inside(o.f).sum()))
         if n in every_steps:
             snap(n)
+            # the run so far, so that an interrupted run still compares up to here
+            np.savez_compressed(out, f_final=o.f, last_step=n, **{k: np.array(v) for k, v in rec.items()})
         if n % 10 == 0 or n in every_steps:
             say(f"oracle step {n}/{len(dts)}: {time.perf_counter() - t0:.0f} s")
-    np.savez_compressed(out, f_final=o.f, **{k: np.array(v) for k, v in rec.items()})
+    np.savez_compressed(out, f_final=o.f, last_step=len(dts), **{k: np.array(v) for k, v in rec.items()})
 
 
 def bubbles_of(f):
@@ -129,12 +131,15 @@ def bubbles_of(f):
 
 def compare(dev, orc):
     d, o = np.load(dev), np.load(orc)
+    last = int(o["last_step"]) if "last_step" in o else len(d["dt"])
+    if last < len(d["dt"]):
+        print(f"the oracle run stopped at step {last}: the final fields are compared with the device's end")
     print("step | " + " | ".join(INTEGRALS) + "  (device / oracle - 1)")
-    for k, n in enumerate(d["metric_steps"]):
+    for k, n in enumerate(d["metric_steps"][:len(o["metric_steps"])]):
         a, b = d["metrics"][k], o["metrics"][k]
         rel = [(x - y) / (abs(y) if abs(y) > 0 else 1.0) for x, y in zip(a, b)]
         print(f"{int(n):4d} | " + " | ".join(f"{x:.3e}/{r:+.1e}" for x, r in zip(b, rel)))
-    vd, vo = d["vol"], o["vol"]
+    vd, vo = d["vol"][:len(o["vol"])], o["vol"]
     print(f"Σf drift over the run: device {(vd[-1] - vd[0]) / vd[0]:+.2e}, oracle {(vo[-1] - vo[0]) / vo[0]:+.2e}")
     fd, fo = d["f_final"].astype(np.float64), o["f_final"]
     diff = np.abs(fd - fo)[1:-1, 1:-1, 1:-1]
