@@ -2,9 +2,10 @@
  * ia.h -- C ABI of the MI355X (gfx950) implementation of the two-phase time
  * step of InterfaceAdvection.jl.
  *
- * A Julia host binds these with `ccall` (see the method table in the
- * integration notes); a Python host binds them with ctypes
- * (interfaceadvection.jl_amd/__init__.py).  Plain pointers and sizes only.
+ * A Julia host binds these with `ccall` from a package extension whose
+ * methods of MPFMomStep! and advect! dispatch on device-array fields; a
+ * Python host binds them with ctypes (interfaceadvection.jl_amd/__init__.py).
+ * Plain pointers and sizes only.
  *
  * Layout of every field buffer handed across this boundary is exactly the
  * reference's Julia array: column-major, x fastest, one ghost layer per side
