@@ -104,12 +104,13 @@ def oracle(dev, out, threads):
         o.Δt[-1] = np.float64(dts[n - 1])
         o.sim_step()
         rec["vol"].append(float(O.inside(o.f).sum()))
+        rec["iters"].append([o.lib.oc_last_iters(o.ctx, 0), o.lib.oc_last_iters(o.ctx, 1)])
         if n in every_steps:
             snap(n)
             # the run so far, so that an interrupted run still compares up to here
             np.savez_compressed(out, f_final=o.f, last_step=n, **{k: np.array(v) for k, v in rec.items()})
         if n % 10 == 0 or n in every_steps:
-            say(f"oracle step {n}/{len(dts)}: {time.perf_counter() - t0:.0f} s")
+            say(f"oracle step {n}/{len(dts)}: {time.perf_counter() - t0:.0f} s, FCG its {rec['iters'][-1]}")
     np.savez_compressed(out, f_final=o.f, last_step=len(dts), **{k: np.array(v) for k, v in rec.items()})
 
 
@@ -139,6 +140,10 @@ def compare(dev, orc):
         a, b = d["metrics"][k], o["metrics"][k]
         rel = [(x - y) / (abs(y) if abs(y) > 0 else 1.0) for x, y in zip(a, b)]
         print(f"{int(n):4d} | " + " | ".join(f"{x:.3e}/{r:+.1e}" for x, r in zip(b, rel)))
+    if len(o["iters"]):
+        di, oi = d["iters"][:len(o["iters"])], o["iters"]
+        print(f"flexible-CG iterations per solve: device {di.mean():.2f} (max {di.max()}), "
+              f"oracle {oi.mean():.2f} (max {oi.max()})")
     vd, vo = d["vol"][:len(o["vol"])], o["vol"]
     print(f"Σf drift over the run: device {(vd[-1] - vd[0]) / vd[0]:+.2e}, oracle {(vo[-1] - vo[0]) / vo[0]:+.2e}")
     fd, fo = d["f_final"].astype(np.float64), o["f_final"]
