@@ -1,12 +1,14 @@
-"""The 256³ pool (32 bubbles of radius U(0.03, 0.08) L, seed 3; slip walls,
-gravity, density ratio 1000, viscosity ratio 100) at its full length: 500
-REFINED steps on the device against the FP64 oracle run over the same Δt
-sequence from the same initial f.  The step is MPFMomStep! (flow.jl:60-109);
-the integrals are metrics.jl:15-50 (kinetic energy, momentum, potential
-energy, enstrophy, Σf).
+"""The pool (32 bubbles of radius U(0.03, 0.08) L, seed 3; slip walls,
+gravity, density ratio 1000, viscosity ratio 100) over many steps: REFINED
+steps on the device against the FP64 oracle run over the same Δt sequence
+from the same initial f.  The step is MPFMomStep! (flow.jl:60-109); the
+integrals are metrics.jl:15-50 (kinetic energy, momentum, potential energy,
+enstrophy, Σf).  The grid side is 256 (the benchmark case), or C3_N.
 
-Not collected by pytest: the oracle leg takes about an hour and a half on
-eight cores.  Three stages, the first on a GPU, the others on any CPU:
+Not collected by pytest: the FP64 oracle takes about 30 s per 256³ step on
+eight cores, and several times that where the solves get hard, so the
+oracle leg of a long run takes hours.  Three stages, the first on a GPU,
+the others on any CPU:
 
     python tests/c3_trajectory.py device DEV.npz [--steps 500 --every 50]
     python tests/c3_trajectory.py oracle DEV.npz ORC.npz
