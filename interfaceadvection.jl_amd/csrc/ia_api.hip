@@ -64,6 +64,8 @@ struct Level {
     void *L = nullptr, *Dg = nullptr, *iD = nullptr, *x = nullptr, *e = nullptr, *r = nullptr, *z = nullptr;
     void *e2 = nullptr, *z2 = nullptr;  // second ϵ/z pair of the fused pcg! iterations (levels >= 1)
     void *r2 = nullptr, *w = nullptr, *w2 = nullptr;  // single-reduction CG: r, w pairs (levels >= 1)
+    CTiling wt;      // the single-reduction CG step's tiling on one-cell-per-thread levels (wnb threads)
+    int wnb = NT;
     bool own = false;
 };
 
@@ -104,6 +106,7 @@ struct ia_ctx {
     // PLAIN), and fcg_coarse_its(level cells) in the flexible CG's preconditioner
     // (IA_FCG_CIT=a,b,... overrides levels 1, 2, ...; the last value repeats)
     int cit_wl[MAXLEV], cit_fcg[MAXLEV];
+    int cg1_nb[MAXLEV];  // workgroup size of the coarse CG step per level (one cell per thread)
     struct Cnn {
         bool ready = false;
         void *act[CNN_L + 1] = {};   // layer inputs/outputs, BF16 channels-last
@@ -465,6 +468,13 @@ template <class T> struct Impl {
             grid_init(lv.g, g.D, Na, c->P.perdir);
             lv.t = tiling(lv.g);
             lv.ct = ctiling(lv.g, true);
+            lv.wt = lv.ct;
+            lv.wnb = c->cg1_nb[c->nlev];
+            if (lv.ct.zb == 1 && lv.g.D == 3) {
+                int n1 = lv.g.D > 1 ? lv.g.Ng[1] - 2 : 1, rows = lv.wnb / lv.wt.nx;
+                lv.wt.nt[1] = (n1 + rows - 1) / rows;
+                lv.wt.nblk = (long)lv.wt.nt[0] * lv.wt.nt[1] * lv.wt.nt[2];
+            }
             size_t n = lv.g.ns;
             if (int r = alloc_zero(c, &lv.L, n * g.D * es)) return r;
             void **lb[] = {&lv.Dg, &lv.iD, &lv.e, &lv.r, &lv.x, &lv.z, &lv.e2, &lv.z2, &lv.r2, &lv.w, &lv.w2};
@@ -614,8 +624,24 @@ template <class T> struct Impl {
             Cg1V<T> b{r[a], w[a], s[a], r[1 - a], w[1 - a], s[1 - a]};
             double *pin = pp(c, (q & 1) ? PP_A : PP_B), *pout = pp(c, (q & 1) ? PP_B : PP_A);
             if (l.ct.zb == 1) {
-                auto k = l.g.D == 3 ? k_cg1_step1<3, T> : k_cg1_step1<2, T>;
-                hipLaunchKernelGGL(k, B, dim3(NT), 0, c->s, v, b, c->pst, q, it, pin, pout);
+                LevV<T> vw = v;
+                vw.t = l.wt;
+                long nin = q == 1 ? l.ct.nblk : l.wt.nblk;
+                dim3 W((unsigned)l.wt.nblk);
+                if (l.g.D == 2)
+                    hipLaunchKernelGGL((k_cg1_step1<2, T>), B, dim3(NT), 0, c->s, v, b, c->pst, q, it, pin, nin, pout);
+                else if (l.wnb == 1024)
+                    hipLaunchKernelGGL((k_cg1_step1<3, T, 1024>), W, dim3(1024), 0, c->s, vw, b, c->pst, q, it, pin,
+                                       nin, pout);
+                else if (l.wnb == 128)
+                    hipLaunchKernelGGL((k_cg1_step1<3, T, 128>), W, dim3(128), 0, c->s, vw, b, c->pst, q, it, pin, nin,
+                                       pout);
+                else if (l.wnb == 512)
+                    hipLaunchKernelGGL((k_cg1_step1<3, T, 512>), W, dim3(512), 0, c->s, vw, b, c->pst, q, it, pin, nin,
+                                       pout);
+                else
+                    hipLaunchKernelGGL((k_cg1_step1<3, T>), W, dim3(NT), 0, c->s, vw, b, c->pst, q, it, pin, nin,
+                                       pout);
             } else {
                 hipLaunchKernelGGL(k_cg1_step<T>, B, dim3(NT), 0, c->s, v, b, c->pst, q, it, pin, pout);
             }
@@ -1662,6 +1688,23 @@ int ia_create(const ia_params *p, ia_ctx **out) {
         for (int a = 0; a < p->D; ++a) n *= std::max(1L, (long)(p->N[a] >> l));
         c->cit_wl[l] = 6;
         c->cit_fcg[l] = fcg_coarse_its(n, l);
+        c->cg1_nb[l] = p->D == 3 ? cg1_workgroup(n) : NT;
+    }
+    // IA_CG1_NB=a,b,...: the coarse CG step's workgroup size on levels 1, 2, ...
+    // (128, 256, 512 or 1024; used where a level has one cell per thread)
+    if (const char *nv = getenv("IA_CG1_NB"); nv && *nv) {
+        int l = 1;
+        const char *q = nv;
+        while (*q && l < MAXLEV) {
+            char *end;
+            int v = (int)strtol(q, &end, 10);
+            if (end == q || (v != 128 && v != 256 && v != 512 && v != 1024)) {
+                delete c;
+                return IA_ERR_BADARG;
+            }
+            c->cg1_nb[l++] = v;
+            q = *end == ',' ? end + 1 : end;
+        }
     }
     if (const char *cv = getenv("IA_FCG_CIT"); cv && *cv) {
         int l = 1, v = 0;

@@ -940,8 +940,9 @@ __global__ void __launch_bounds__(NT) k_cg1_step(LevV<T> l, Cg1V<T> b, PcgState 
 // block_sum / block_total, so the same values)
 // lead: the barrier that protects sh2 from an earlier block_sum2 of the
 // same kernel (not needed on its first use)
+template <int NB = NT>
 __device__ inline void block_sum2(double &a, double &b, bool lead = true) {  // valid in thread 0
-    __shared__ double sh2[2][NT / 64];
+    __shared__ double sh2[2][NB / 64];
     a = wave_sum(a);
     b = wave_sum(b);
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -953,7 +954,7 @@ __device__ inline void block_sum2(double &a, double &b, bool lead = true) {  // 
     __syncthreads();
     if (threadIdx.x == 0) {
         double sa = 0, sb = 0;
-        for (int q = 0; q < NT / 64; ++q) {
+        for (int q = 0; q < NB / 64; ++q) {
             sa += sh2[0][q];
             sb += sh2[1][q];
         }
@@ -982,19 +983,35 @@ __device__ inline void block_total2(const double *part, long n, double &a, doubl
 // the thread needs -- the previous partials, and r, w, s, iD at the cell and
 // its neighbours -- is issued before the partial sums are reduced, so the
 // launch waits for one memory round trip instead of three.  Same arithmetic.
-template <int D, class T>
-__global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState *S, int q, int it,
-                                                  const double *pin, double *pout) {
+// NB threads per workgroup (l.t tiles rows of nx cells, NB/nx rows per
+// workgroup); nin partial pairs come in (k_cg1_init's, then this kernel's).
+template <int NB>
+__device__ inline bool thread_row(const Grid &g, const CTiling &t, int p[3]) {
+    long b = swizzle_block((long)blockIdx.x, t.nblk);
+    long bx = b % t.nt[0];
+    long r = b / t.nt[0];
+    long by = r % t.nt[1];
+    long bz = r / t.nt[1];
+    int tx = threadIdx.x % t.nx, ty = threadIdx.x / t.nx;
+    p[0] = 1 + (int)bx * t.nx + tx;
+    p[1] = g.D > 1 ? 1 + (int)by * (NB / t.nx) + ty : 0;
+    p[2] = g.D > 2 ? 1 + (int)bz : 0;
+    bool ok = p[0] <= g.Ng[0] - 2;
+    if (g.D > 1) ok = ok && p[1] <= g.Ng[1] - 2;
+    return ok;
+}
+template <int D, class T, int NB = NT>
+__global__ void __launch_bounds__(NB) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState *S, int q, int it,
+                                                  const double *pin, long nin, double *pout) {
     HALT_IF(l)
     const Grid &g = l.g;
-    int p[3], zlo, zhi;
-    bool act = thread_col(g, l.t, p, zlo, zhi);
-    p[2] = zlo;
-    long I = (long)p[0] + g.st[1] * p[1] + g.st[2] * zlo;
+    int p[3];
+    bool act = thread_row<NB>(g, l.t, p);
+    long I = (long)p[0] + g.st[1] * p[1] + g.st[2] * p[2];
     PcgState st = S[q - 1];
     // the previous launch's partials (strided per thread, as block_total)
     double va = 0, vb = 0;
-    for (long k = threadIdx.x; k < l.t.nblk; k += NT) {
+    for (long k = threadIdx.x; k < nin; k += NB) {
         va += pin[2 * k];
         vb += pin[2 * k + 1];
     }
@@ -1029,7 +1046,7 @@ __global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState
     if (reduce) {
         // block_total2, with the per-thread sums already loaded; every
         // thread sums the wavefronts' sums in order (one barrier)
-        __shared__ double shs[2][NT / 64];
+        __shared__ double shs[2][NB / 64];
         va = wave_sum(va);
         vb = wave_sum(vb);
         if ((threadIdx.x & 63) == 0) {
@@ -1038,8 +1055,8 @@ __global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState
         }
         __syncthreads();
         double sa = 0, sb = 0;
-        for (int w = 0; w < NT / 64; ++w) sa += shs[0][w];
-        for (int w = 0; w < NT / 64; ++w) sb += shs[1][w];
+        for (int w = 0; w < NB / 64; ++w) sa += shs[0][w];
+        for (int w = 0; w < NB / 64; ++w) sb += shs[1][w];
         T gam = (T)sa;
         T del = (T)sb;
         if (fabs(gam) < 10 * eps_of<T>()) {
@@ -1091,7 +1108,7 @@ __global__ void __launch_bounds__(NT) k_cg1_step1(LevV<T> l, Cg1V<T> b, PcgState
         }
     }
     if (st.stop) return;
-    block_sum2(gd, dd, false);  // sh2's first use in this kernel
+    block_sum2<NB>(gd, dd, false);  // sh2's first use in this kernel
     if (threadIdx.x == 0) {
         pout[2 * blockIdx.x] = gd;
         pout[2 * blockIdx.x + 1] = dd;
@@ -1702,6 +1719,12 @@ constexpr int PCG_SLOTS = 3 * CIT_MAX + 8;
 __host__ __device__ inline int fcg_coarse_its(long n, int l) {
     return (l == 1 && n > 1000000) ? 6 : n > 100000 ? 56 : n > 512 ? 64 : 32;
 }
+// workgroup size of the single-reduction CG step on a 3D level of n cells
+// with one cell per thread: 512 above 100k cells (64³: 512 workgroups, so
+// each re-reads half as many partial pairs; 6.3 µs a step against 6.9 with
+// 256 threads, 6.5 with 1024), else 256 (32³: 4.4 µs, against 4.7 with 512
+// and 4.5 with 128)
+inline int cg1_workgroup(long n) { return n > 100000 ? 512 : NT; }
 constexpr size_t TAIL_LDS_MAX = 152 * 1024;  // of the 160 KiB of LDS per CU
 template <class T> struct TailLev {
     Grid g;
