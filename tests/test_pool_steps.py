@@ -20,6 +20,8 @@ residual 1e-6), the mode the benchmark times.
 - 64³ against the FP64 oracle from the same FP32 inputs (the error of the
   FP32 stencils): f within 2e-5 (measured 1.7e-6 and 2.9e-6) and u within
   1e-3 of max|u| (measured 1.1e-4 and 1.2e-4) after 1 and 5 steps.
+- 128³: one step with the coarse CG step's workgroups at the default sizes
+  and at three other sizes per level, each against the FP32 oracle.
 - 256³, the benchmark case itself (32 bubbles, seed 3): five steps with
   property checks: status OK, f ∈ [0, 1], Σf (FP64 on the device) drifting by
   at most 1e-8 relative (measured 1.3e-10 after 5 steps), every solve at or
@@ -167,6 +169,54 @@ def test_pool64_fp32_error_vs_fp64_oracle():
             assert df <= 2e-5
             assert du <= 1e-3
     g.close()
+
+
+def test_pool128_coarse_cg_workgroup_sizes_match_oracle():
+    """The coarse CG step's workgroup size per level (IA_CG1_NB: 512 threads
+    on levels above 100k cells, else 256, by default) changes only the order
+    in which its partial sums add.  At 128³ the 64³ and 32³ levels run the
+    one-cell-per-thread step; one whole REFINED step with the default sizes
+    and with 1024/128, 256/256 and 128/1024 threads on those levels, each
+    against the same FP32 oracle step, with the 64³ test's bars."""
+    ref = None
+    for nb in (None, "1024,128", "256,256", "128,1024"):  # levels 1 (64³) and 2 (32³)
+        say("pool 128^3 REFINED, one step, IA_CG1_NB =", nb or "default")
+        env_old = os.environ.get("IA_CG1_NB")
+        if nb is not None:
+            os.environ["IA_CG1_NB"] = nb
+        try:
+            g, _, kw = pool.build_sim(ia, 128, "refined", 1e-6)
+        finally:
+            if env_old is None:
+                os.environ.pop("IA_CG1_NB", None)
+            else:
+                os.environ["IA_CG1_NB"] = env_old
+        if ref is None:
+            okw = dict(kw)
+            okw["T"] = np.float32
+            o = O.OracleSim((128,) * 3, (0.0, 0.0, 0.0), 128.0, poisson="refined", poisson_rtol=1e-6, **okw)
+            f_in = g.intf.f.copy()
+            o.set(O.F, f_in)
+            o.set(O.F0, g.intf.f0)
+            o.Δt = [o.T.type(g.flow.Δt[-1])]
+            o.sim_step()
+            ref = (o.f.astype(np.float64), np.asarray(o.u, dtype=np.float64),
+                   [o.lib.oc_last_iters(o.ctx, 0), o.lib.oc_last_iters(o.ctx, 1)])
+            del o
+        else:
+            assert np.array_equal(g.intf.f, f_in)
+        st = ia.sim_step_(g)
+        gi = [st.solve[0].iters, st.solve[1].iters]
+        df = float(np.abs(g.intf.f.astype(np.float64) - ref[0]).max())
+        du = rel_u(g.flow.u, ref[1])
+        say(f"max|Δf| {df:.3e}, max|Δu|/max|u| {du:.3e}, FCG its gpu {gi} oracle {ref[2]}")
+        assert st.status == 0
+        assert df <= 1e-6
+        assert du <= 1e-5
+        for q in range(2):
+            assert 0 < st.solve[q].rel <= 1e-6
+            assert abs(gi[q] - ref[2][q]) <= 1
+        g.close()
 
 
 def test_pool256_refined_properties():
