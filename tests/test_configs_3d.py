@@ -8,7 +8,7 @@
   the device) within 1e-8 relative of its start, every solve at or below 1e-6,
   and the learned network evaluated on every LEARNED solve.
 - The learned correction on a layout it was not trained on (the 256³ pool,
-  seed 3; training used seeds 1000-1003): over the same steps, LEARNED needs
+  seed 3; training used seeds 1000-1044): over the same steps, LEARNED needs
   no more flexible-CG iterations than REFINED, and every accepted correction
   lowered the FP64 residual (the acceptance test).  The times are printed:
   on this case the network's inference costs about what the iteration it
