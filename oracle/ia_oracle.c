@@ -326,7 +326,9 @@ static void normal_column(const grid_t *g, const T *f, T *nh, size_t I) {
 }
 /* normalEstimation.jl:105-154 (default scheme).  Float64 literals promote
  * exactly as in Julia.  pinned by: planar interfaces from the exact forward
- * PLIC (tests/test_oracle_normals.py) -- generic blocks are parity unpinned. */
+ * PLIC in both regimes, and first-order convergence on circles and spheres
+ * (tests/test_oracle_pins.py) -- exact values on generic blocks are parity
+ * unpinned. */
 static void normal_wh(const grid_t *g, const T *f, T *nh, size_t I) {
     normal_column(g, f, nh, I);
     int im = argabsmax(g, nh, I);

@@ -4,7 +4,8 @@ Each check is chosen so that a plausible slip in the restatement (a dropped
 term, a wrong sign or index) fails it:
 - PLIC forward against an independent closed form (inclusion-exclusion over
   the cube's vertices) and the inverse through round trips;
-- WH normal on exact planar blocks in its column regime;
+- WH normal on exact planar blocks in its column regime and its row-width
+  branches, and its convergence on circles and spheres;
 - MultiLevelPoisson's converged solution against a sparse direct solve;
 - CMOM consistency: a uniform velocity stays uniform across density jumps;
 - cVOF: Σf conserved to round-off under a discretely divergence-free field,
@@ -178,6 +179,101 @@ def test_wh_normal_row_width_regime(D):
             assert np.allclose(out, n, atol=1e-12), (kind, s, h, fx, fy, tr, out, n)
         hits[kind] += 1
     assert hits["under"] > 20 and hits["over"] > 20, hits
+
+
+def _disc_fractions(R, c, n):
+    """Exact fractions of the disc |x - c| <= R in the cells [i, i+1] x [j, j+1]
+    of an n x n block: the chord length integrated over x by adaptive
+    quadrature, with the points where the circle crosses the cell's rows as
+    breakpoints."""
+    from scipy import integrate
+    f = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            dx, dy = abs(i + 0.5 - c[0]), abs(j + 0.5 - c[1])
+            if math.hypot(dx + 0.5, dy + 0.5) <= R:
+                f[i, j] = 1.0
+                continue
+            if math.hypot(max(dx - 0.5, 0.0), max(dy - 0.5, 0.0)) >= R:
+                continue
+            a, b = max(i, c[0] - R), min(i + 1, c[0] + R)
+
+            def chord(x):
+                h = math.sqrt(max(R * R - (x - c[0]) ** 2, 0.0))
+                return max(min(c[1] + h, j + 1) - max(c[1] - h, j), 0.0)
+
+            pts = [c[0] + sg * math.sqrt(R * R - (y - c[1]) ** 2) for y in (j, j + 1) for sg in (-1, 1)
+                   if R * R > (y - c[1]) ** 2]
+            pts = sorted(x for x in pts if a < x < b)
+            f[i, j] = integrate.quad(chord, a, b, points=pts or None, epsabs=1e-13, epsrel=1e-13, limit=200)[0]
+    return f
+
+
+def _ball_fractions(R, c, n, m=64):
+    """Fractions of the ball |x - c| <= R in the cells of an n³ block: the
+    z-extent inside the cell exact, averaged over m x m midpoints in x, y
+    (error about 1e-4, against the normals' errors of order 1e-2)."""
+    f = np.zeros((n, n, n))
+    u = (np.arange(m) + 0.5) / m
+    X, Y = np.meshgrid(u, u, indexing="ij")
+    for i, j, k in itertools.product(range(n), repeat=3):
+        d = np.abs(np.array([i, j, k]) + 0.5 - c)
+        if np.linalg.norm(d + 0.5) <= R:
+            f[i, j, k] = 1.0
+            continue
+        if np.linalg.norm(np.maximum(d - 0.5, 0.0)) >= R:
+            continue
+        r2 = (i + X - c[0]) ** 2 + (j + Y - c[1]) ** 2
+        h = np.sqrt(np.maximum(R * R - r2, 0.0))
+        ext = np.minimum(c[2] + h, k + 1) - np.maximum(c[2] - h, k)
+        f[i, j, k] = np.where(r2 < R * R, np.maximum(ext, 0.0), 0.0).mean()
+    return f
+
+
+def _wh_angle_errors(f, c):
+    """Angle between the WH normal of every mixed cell and the radial
+    direction through the cell's centre (the exact normal there, pointing
+    out of the dark region as WH's does)."""
+    D = f.ndim
+    L = O.lib(np.float64)
+    Ng, I = O._ints([3] * D), O._ints([2] * D)
+    err = []
+    for J in itertools.product(*[range(1, n - 1) for n in f.shape]):
+        if not (1e-9 < f[J] < 1 - 1e-9):
+            continue
+        blk = np.asfortranarray(f[tuple(slice(j - 1, j + 2) for j in J)])
+        out = np.zeros(D)
+        L.oc_normal_block(D, Ng, O._ptr(blk), O.NORMAL["WH"], I, O._ptr(out))
+        ex = np.array(J) + 0.5 - c
+        cosang = out @ ex / (np.linalg.norm(out) * np.linalg.norm(ex))
+        err.append(math.acos(min(1.0, max(-1.0, cosang))))
+    return np.array(err)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_wh_normal_converges_on_curved_interfaces(D):
+    """WH on generic, curved blocks: the normal of a consistent estimator
+    converges to the exact one as the radius of curvature R grows (in cells).
+    On circles WH's worst cell is within 0.4/R rad (measured 0.31-0.33/R for
+    R = 8, 16, 32); on spheres, where a few cells near the diagonals keep
+    errors of 0.15-0.3 rad, the RMS over the mixed cells is within 0.25/R
+    (measured 0.18-0.19/R for R = 4, 8, 16).  Both halve when R doubles.
+    Mistakes planted in the oracle's WH (a sign in a row-width ratio, hl for
+    hr, the offsets δd or δ(dominant) reversed, the column slope not halved)
+    give 1.0-4/R or worse, growing with R."""
+    rng = np.random.default_rng(7 + D)
+    radii = (8, 16, 32) if D == 2 else (4, 8, 16)
+    stats = []
+    for R in radii:
+        n = int(2 * R + 6)
+        c = n / 2 + rng.uniform(-0.5, 0.5, D)
+        f = _disc_fractions(R, c, n) if D == 2 else _ball_fractions(R, c, n)
+        e = _wh_angle_errors(f, c)
+        assert len(e) > 4 * R
+        stat = e.max() if D == 2 else math.sqrt((e ** 2).mean())
+        assert stat * R <= (0.4 if D == 2 else 0.25), (R, stat)
+        stats.append(stat)
+    assert stats[-1] <= 0.35 * stats[0], stats
 
 
 def _poisson_matrix(sim):
