@@ -54,7 +54,10 @@ enum ia_limiter {
  * REFINED:   FP64 flexible CG (FCG(1)) preconditioned by one FP32 V-cycle (+ two
  *            Chebyshev smoothing steps on the finest level) per iteration;
  *            same stopping test, FP64 residual and pressure accumulation.  No
- *            reference counterpart.
+ *            reference counterpart.  Read at ia_create, for tuning sweeps:
+ *            IA_FCG_CIT=a,b,... (coarse iterations on levels 1, 2, ...),
+ *            IA_FCG_SMOOTH=n (n fine pcg! iterations instead of Chebyshev),
+ *            IA_CG1_NB=a,b,... (coarse CG workgroup sizes on levels 1, 2, ...).
  * PCG:       inproject!(a, b::Poisson, dt) (flow.jl:337-341): psolver! (flow.jl:300-326),
  *            Jacobi-preconditioned CG on the single (finest) level until the
  *            absolute Σr² <= 50eps(T) or 2000 iterations, as the reference runs it
