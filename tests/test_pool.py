@@ -1,7 +1,11 @@
 """The pool workload generator (interfaceadvection.jl_amd/pool.py): bubble
-placement constraints, and the truncated SDF sampling, whose applyVOF!
-fractions must equal those of the full SDF max(z - zmax L, max_b(r_b - |x - c_b|))."""
+placement constraints, the committed bubble lists (tests/golden/bubbles_*.json,
+written by tests/golden/make_bubble_lists.py), and the truncated SDF sampling,
+whose applyVOF! fractions must equal those of the full SDF
+max(z - zmax L, max_b(r_b - |x - c_b|))."""
 import importlib
+import json
+import os
 
 import numpy as np
 import pytest
@@ -28,6 +32,19 @@ def test_bubbles_respect_clearances(name):
     np.fill_diagonal(d, np.inf)
     assert (d >= r[:, None] + r[None, :] + 2).all()
     assert pool.bubbles(N, c["nb"], c["seed"], c["rmin"], c["rmax"], c["zmax"]) == bl
+
+
+@pytest.mark.parametrize("name", sorted(pool.CASES))
+def test_bubbles_match_the_committed_lists(name):
+    """The workload is its bubble list: the generator (seeded PCG64) must
+    reproduce the committed one exactly, so that benchmark and parity runs on
+    any numpy build advance the same state."""
+    c = pool.CASES[name]
+    with open(os.path.join(os.path.dirname(__file__), "golden", f"bubbles_{name}.json")) as fh:
+        rec = json.load(fh)
+    assert (rec["N"], rec["seed"]) == (c["N"], c["seed"])
+    bl = pool.bubbles(c["N"], c["nb"], c["seed"], c["rmin"], c["rmax"], c["zmax"])
+    assert [list(map(float, b)) for b in bl] == rec["bubbles"]
 
 
 def full_sdf_samples(N, bl, zmax, T=np.float32):
