@@ -1782,6 +1782,33 @@ int oc_mg_level_dims(oc_ctx *c, int l, int *Ng) {
     for (int d = 0; d < 3; ++d) Ng[d] = c->lev[l].g.Ng[d];
     return OC_OK;
 }
+/* a copy of level l's array (ghost-inclusive): which 0 = L (D components),
+ * 1 = D (diagonal), 2 = iD, 3 = r, 4 = x, 5 = ϵ.  For the tests of the
+ * multigrid's transfer operators and coarse coefficients. */
+int oc_mg_level_get(oc_ctx *c, int l, int which, real_t *out) {
+    if (l < 0 || l >= c->nlev || which < 0 || which > 5) return OC_BADARG;
+    lev_t *lv = &c->lev[l];
+    size_t n = lv->g.ns * (which == 0 ? (size_t)lv->g.D : 1);
+    const T *a = which == 0 ? lv->L : which == 1 ? lv->Dg : which == 2 ? lv->iD : which == 3 ? lv->r
+                                                                    : which == 4 ? lv->x : lv->eps;
+    for (size_t q = 0; q < n; ++q) out[q] = a[q];
+    return OC_OK;
+}
+/* level l's r set from `in`, then restrict! into level l+1 (whose r is
+ * returned by oc_mg_level_get), or prolongate! level l+1's x (set from `in`)
+ * into level l's ϵ: dir 0 restricts, 1 prolongates. */
+int oc_mg_transfer(oc_ctx *c, int l, int dir, const real_t *in) {
+    if (l < 0 || l + 1 >= c->nlev) return OC_BADARG;
+    lev_t *fl = &c->lev[l], *co = &c->lev[l + 1];
+    if (dir == 0) {
+        for (size_t q = 0; q < fl->g.ns; ++q) fl->r[q] = in[q];
+        mg_restrict(co, fl);
+    } else {
+        for (size_t q = 0; q < co->g.ns; ++q) co->x[q] = in[q];
+        mg_prolongate(fl, co);
+    }
+    return OC_OK;
+}
 const char *oc_last_error(oc_ctx *c) { return c->msg; }
 void oc_set_threads(int n) {
 #ifdef _OPENMP

@@ -68,6 +68,8 @@ int oc_set_field(oc_ctx *c, int id, const void *src);
 int oc_get_field(oc_ctx *c, int id, void *dst);
 int oc_mg_nlevels(oc_ctx *c);
 int oc_mg_level_dims(oc_ctx *c, int l, int *Ng);
+int oc_mg_level_get(oc_ctx *c, int l, int which, real_t *out);
+int oc_mg_transfer(oc_ctx *c, int l, int dir, const real_t *in);
 
 /* construction helpers */
 int oc_apply_vof(oc_ctx *c, const real_t *sdf);   /* applyVOF! + cleanWisp! + BCf! + f0=f */

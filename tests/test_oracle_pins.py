@@ -278,10 +278,13 @@ def test_wh_normal_converges_on_curved_interfaces(D):
 
 def _poisson_matrix(sim):
     """Assemble A of WaterLily's mult(I,L,D,x) on the interior from μ₀ (levels[1].L)."""
-    L = sim.get(O.MU0)
-    D = sim.D
-    Ng = sim.Ng
-    N = sim.dims
+    return _mult_matrix(sim.get(O.MU0), sim.dims)
+
+
+def _mult_matrix(L, N):
+    """A of mult(I,L,D,x) with D = -Σ(L[I] + L[I+δ]) on the interior cells of
+    a ghost-inclusive face-coefficient array L (periodic images wrap)."""
+    D = len(N)
     idx = np.arange(np.prod(N)).reshape(N, order="F")
     rows, cols, vals = [], [], []
     for I in itertools.product(*[range(1, n + 1) for n in N]):
@@ -304,6 +307,69 @@ def _poisson_matrix(sim):
         vals.append(diag)
     n = int(np.prod(N))
     return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def _level(o, l, which):
+    ng = O._ints([0, 0, 0])
+    assert o.lib.oc_mg_level_dims(o.ctx, l, ng) == 0
+    Ng = tuple(ng[:o.D])
+    a = np.zeros(Ng + ((o.D,) if which == 0 else ()), dtype=np.float64, order="F")
+    assert o.lib.oc_mg_level_get(o.ctx, l, which, O._ptr(a)) == 0
+    return a
+
+
+def _parent_matrix(N):
+    """P: fine interior cell -> its parent (prolongate!'s down(I) = (I+2)÷2,
+    1-based and ghost-inclusive, is the coarse cell ⌈i/2⌉ of interior cell i)."""
+    Nc = tuple(n // 2 for n in N)
+    fine = np.arange(int(np.prod(N))).reshape(N, order="F")
+    coarse = np.arange(int(np.prod(Nc))).reshape(Nc, order="F")
+    rows, cols = [], []
+    for I in itertools.product(*[range(n) for n in N]):
+        rows.append(fine[I])
+        cols.append(coarse[tuple(i // 2 for i in I)])
+    return sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(fine.size, coarse.size)), Nc
+
+
+@pytest.mark.parametrize("D,perdir", [(2, ()), (2, (1,)), (3, ()), (3, (2, 3))])
+def test_coarse_levels_are_half_the_galerkin_operator(D, perdir):
+    """MultiLevelPoisson's transfers and coarse coefficients (WaterLily
+    restrict!, prolongate!, restrictL!, and the diagonal D = -Σ L): with P the
+    piecewise-constant prolongation, prolongate! is P, restrict! is Pᵀ, and
+    every coarse level's operator is ½ PᵀAP of the level above it, coefficient
+    by coefficient, on a two-phase field (density ratio 1000) with walls and
+    periodic axes.  A coarse face coefficient summed over the wrong fine
+    faces, a missing ½, or a restriction over the wrong children breaks it."""
+    N = 16
+    rng = np.random.default_rng(21 + D)
+    c = rng.uniform(5, 11, D)
+    sdf = lambda x: np.sqrt(sum((x[a] - c[a]) ** 2 for a in range(D))) - 3.7
+    o = O.OracleSim((N,) * D, (0.0,) * D, float(N), T=np.float64, λρ=1e-3, InterfaceSDF=sdf, perdir=perdir)
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    nlev = o.lib.oc_mg_nlevels(o.ctx)
+    assert nlev >= 3
+    dims = (N,) * D
+    for l in range(nlev - 1):
+        Lf, Lc, Dc = _level(o, l, 0), _level(o, l + 1, 0), _level(o, l + 1, 1)
+        P, dc = _parent_matrix(dims)
+        G = (P.T @ _mult_matrix(Lf, dims) @ P).tocsr()
+        H = _mult_matrix(Lc, dc)
+        err = abs(G - 2 * H).max()
+        assert err <= 1e-12 * abs(G).max(), (l, err)
+        assert np.allclose(2 * Dc[(slice(1, -1),) * D].ravel(order="F"), G.diagonal(), rtol=1e-13, atol=1e-14)
+        # restrict! = Pᵀ and prolongate! = P on random fields
+        r = np.zeros(Lf.shape[:D], order="F")
+        r[(slice(1, -1),) * D] = rng.standard_normal(dims)
+        assert o.lib.oc_mg_transfer(o.ctx, l, 0, O._ptr(r)) == 0
+        rc = _level(o, l + 1, 3)[(slice(1, -1),) * D].ravel(order="F")
+        assert np.allclose(rc, P.T @ r[(slice(1, -1),) * D].ravel(order="F"), rtol=1e-14, atol=1e-14)
+        x = np.zeros(Lc.shape[:D], order="F")
+        x[(slice(1, -1),) * D] = rng.standard_normal(dc)
+        assert o.lib.oc_mg_transfer(o.ctx, l, 1, O._ptr(x)) == 0
+        ef = _level(o, l, 5)[(slice(1, -1),) * D].ravel(order="F")
+        assert np.array_equal(ef, P @ x[(slice(1, -1),) * D].ravel(order="F"))
+        dims = dc
 
 
 @pytest.mark.parametrize("perdir", [(), (1,)])
