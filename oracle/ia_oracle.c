@@ -1809,6 +1809,23 @@ int oc_mg_transfer(oc_ctx *c, int l, int dir, const real_t *in) {
     }
     return OC_OK;
 }
+/* level l's array `which` (as oc_mg_level_get; 3 = r, 4 = x only) set from `in` */
+int oc_mg_level_set(oc_ctx *c, int l, int which, const real_t *in) {
+    if (l < 0 || l >= c->nlev || (which != 3 && which != 4)) return OC_BADARG;
+    lev_t *lv = &c->lev[l];
+    T *a = which == 3 ? lv->r : lv->x;
+    for (size_t q = 0; q < lv->g.ns; ++q) a[q] = in[q];
+    return OC_OK;
+}
+/* pcg!(level l; it) on the level's r and x: WaterLily's two-reduction form
+ * (single = 0) or the single-reduction form of the flexible CG's coarse
+ * levels (single = 1; levels >= 1, whose size the scratch covers). */
+int oc_mg_pcg(oc_ctx *c, int l, int it, int single) {
+    if (l < 0 || l >= c->nlev || it < 1 || (single && l < 1)) return OC_BADARG;
+    if (single) mg_pcg1(&c->lev[l], it, c->cg1_u, c->cg1_w, c->cg1_s);
+    else mg_pcg(&c->lev[l], it);
+    return OC_OK;
+}
 const char *oc_last_error(oc_ctx *c) { return c->msg; }
 void oc_set_threads(int n) {
 #ifdef _OPENMP

@@ -70,6 +70,8 @@ int oc_mg_nlevels(oc_ctx *c);
 int oc_mg_level_dims(oc_ctx *c, int l, int *Ng);
 int oc_mg_level_get(oc_ctx *c, int l, int which, real_t *out);
 int oc_mg_transfer(oc_ctx *c, int l, int dir, const real_t *in);
+int oc_mg_level_set(oc_ctx *c, int l, int which, const real_t *in);
+int oc_mg_pcg(oc_ctx *c, int l, int it, int single);
 
 /* construction helpers */
 int oc_apply_vof(oc_ctx *c, const real_t *sdf);   /* applyVOF! + cleanWisp! + BCf! + f0=f */

@@ -61,6 +61,8 @@ def lib(T):
         "oc_mg_level_dims": (C.c_int, [vp, C.c_int, ip]),
         "oc_mg_level_get": (C.c_int, [vp, C.c_int, C.c_int, vp]),
         "oc_mg_transfer": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+        "oc_mg_level_set": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+        "oc_mg_pcg": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "oc_apply_vof": (C.c_int, [vp, vp]),
         "oc_set_initial_velocity": (C.c_int, [vp, vp]),
         "oc_cfl": (C.c_double, [vp]),

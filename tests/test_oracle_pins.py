@@ -372,6 +372,73 @@ def test_coarse_levels_are_half_the_galerkin_operator(D, perdir):
         dims = dc
 
 
+def _two_phase_levels(D, N, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.3 * N, 0.7 * N, D)
+    sdf = lambda x: np.sqrt(sum((x[a] - c[a]) ** 2 for a in range(D))) - 0.23 * N
+    o = O.OracleSim((N,) * D, (0.0,) * D, float(N), T=np.float64, λρ=1e-3, InterfaceSDF=sdf)
+    o.lib.oc_update_L(o.ctx, 0)
+    o.lib.oc_mg_update(o.ctx)
+    return o, rng
+
+
+def _run_pcg(o, l, r0, it, single):
+    D = o.D
+    z = np.zeros_like(r0)
+    assert o.lib.oc_mg_level_set(o.ctx, l, 3, O._ptr(np.asfortranarray(r0))) == 0
+    assert o.lib.oc_mg_level_set(o.ctx, l, 4, O._ptr(np.asfortranarray(z))) == 0
+    assert o.lib.oc_mg_pcg(o.ctx, l, it, single) == 0
+    return _level(o, l, 4), _level(o, l, 3)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_single_reduction_cg_equals_pcg(D):
+    """The flexible CG's coarse solver (Jacobi-preconditioned CG in the
+    single-reduction form of Chronopoulos and Gear) against WaterLily's pcg!
+    (two reductions per iteration): in exact arithmetic the iterates are the
+    same, so after k = 1..8 iterations from the same r (x = 0) on a two-phase
+    level (density ratio 1000) x and r agree to FP64 round-off (bar 1e-10
+    relative).  A wrong α or β in either recurrence breaks it from k = 2."""
+    o, rng = _two_phase_levels(D, 32 if D == 2 else 16, 40 + D)
+    for l in (1, 2):
+        Lc = _level(o, l, 0)
+        r0 = np.zeros(Lc.shape[:D], order="F")
+        r0[(slice(1, -1),) * D] = rng.standard_normal(tuple(n - 2 for n in Lc.shape[:D]))
+        for k in range(1, 9):
+            xa, ra = _run_pcg(o, l, r0, k, 0)
+            xb, rb = _run_pcg(o, l, r0, k, 1)
+            assert np.abs(xa - xb).max() <= 1e-10 * np.abs(xa).max(), (l, k)
+            assert np.abs(ra - rb).max() <= 1e-10 * np.abs(r0).max(), (l, k)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("single", [0, 1])
+def test_pcg_terminates_on_the_smallest_level(D, single):
+    """CG reaches the exact solution in at most as many steps as the
+    preconditioned operator has distinct non-zero eigenvalues: on the 2^D-cell
+    level of an 8^D grid with walls (one null vector, the constant) at most
+    2^D - 1.  From a consistent right-hand side (Σr = 0), pcg! (and the
+    single-reduction form) with that many iterations leave r at round-off,
+    and further iterations are stopped by the |ρ| < 10eps test without
+    moving x."""
+    o, rng = _two_phase_levels(D, 8, 50 + D)
+    l = o.lib.oc_mg_nlevels(o.ctx) - 1
+    Lc = _level(o, l, 0)
+    assert tuple(n - 2 for n in Lc.shape[:D]) == (2,) * D
+    r0 = np.zeros(Lc.shape[:D], order="F")
+    core = rng.standard_normal((2,) * D)
+    r0[(slice(1, -1),) * D] = core - core.mean()
+    m = 2 ** D - 1
+    x, r = _run_pcg(o, l, r0, m, single)
+    assert np.abs(r).max() <= 1e-12 * np.abs(r0).max()
+    x2, _ = _run_pcg(o, l, r0, m + 5, single)
+    assert np.abs(x2 - x).max() <= 1e-12 * np.abs(x).max()
+    # and x solves A x = r0 (the level's own mult, checked independently)
+    A = _mult_matrix(Lc, (2,) * D).toarray()
+    xi = x[(slice(1, -1),) * D].ravel(order="F")
+    assert np.abs(A @ xi - r0[(slice(1, -1),) * D].ravel(order="F")).max() <= 1e-12 * np.abs(r0).max()
+
+
 @pytest.mark.parametrize("perdir", [(), (1,)])
 def test_poisson_converges_to_direct_solve(perdir):
     """solver! run to a tight tolerance equals a sparse direct solve of the same
